@@ -1,0 +1,912 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the reference relaxation core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (packing_amd/) links, loads or
+ * calls this file; tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * use it as the checker / CPU baseline.  Built with -ffp-contract=off so every
+ * multiply and add rounds on its own, as the textual expressions in the reference do.
+ *
+ * Each function names the reference file:line it restates.  The reference keeps its
+ * CUDA in Python strings; the algorithms here follow those strings step for step in
+ * plain C (one thread's work per call, loops where the reference has threads).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_MAXV 8            /* max vertices per convex piece            */
+#define OR_MAXI 16           /* max vertices of a clipped polygon (2*MAXV) */
+#define OR_MAXP 16           /* max pieces                                 */
+
+typedef struct { double x, y; } d2;
+
+static inline d2 mk(double x, double y) { d2 r; r.x = x; r.y = y; return r; }
+
+/* Geometry passed in from Python: P pieces of up to V vertices (local frame). */
+typedef struct {
+    int P, V;
+    const double *piece_xy;   /* P*V*2 */
+    const int *nverts;        /* P     */
+    int n_outline;
+    const double *outline;    /* n_outline*2 */
+    double cutoff_sq;         /* (2R)^2 */
+} or_geom;
+
+/* ---------------------------------------------------------------------------
+ * Primitives: core/pack_cuda_primitives.py:20-87
+ * ------------------------------------------------------------------------- */
+static inline double cross3(d2 a, d2 b, d2 c) {
+    return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+}
+
+/* p1 + t (p2 - p1) on line q1 q2  (core/pack_cuda_primitives.py:25-41) */
+static d2 line_isect(d2 p1, d2 p2, d2 q1, d2 q2) {
+    double rx = p2.x - p1.x, ry = p2.y - p1.y;
+    double sx = q2.x - q1.x, sy = q2.y - q1.y;
+    double den = rx * sy - ry * sx;
+    double t = ((q1.x - p1.x) * sy - (q1.y - p1.y) * sx) / den;
+    return mk(p1.x + t * rx, p1.y + t * ry);
+}
+
+/* |shoelace area| and d|A|/dv  (core/pack_cuda_primitives.py:48-87) */
+static double poly_area_grad(const d2 *v, int n, d2 *dv) {
+    int i;
+    if (n < 3) {
+        for (i = 0; i < n; ++i) dv[i] = mk(0, 0);
+        return 0.0;
+    }
+    double s = 0.0;
+    for (i = 0; i < n; ++i) {
+        int j = (i + 1) % n;
+        s += v[i].x * v[j].y - v[j].x * v[i].y;
+    }
+    double sa = 0.5 * s;
+    double a = sa >= 0.0 ? sa : -sa;
+    if (a == 0.0) {
+        for (i = 0; i < n; ++i) dv[i] = mk(0, 0);
+        return 0.0;
+    }
+    double sg = sa >= 0.0 ? 1.0 : -1.0;
+    for (i = 0; i < n; ++i) {
+        int ip = (i - 1 + n) % n, in = (i + 1) % n;
+        dv[i].x = sg * 0.5 * (v[in].y - v[ip].y);
+        dv[i].y = sg * 0.5 * (v[ip].x - v[in].x);
+    }
+    return a;
+}
+
+/* Reverse mode of line_isect  (core/pack_cuda_primitives.py:98-194) */
+static void line_isect_bwd(d2 p1, d2 p2, d2 q1, d2 q2, d2 g,
+                           d2 *gp1, d2 *gp2, d2 *gq1, d2 *gq2) {
+    double rx = p2.x - p1.x, ry = p2.y - p1.y;
+    double sx = q2.x - q1.x, sy = q2.y - q1.y;
+    double den = rx * sy - ry * sx;
+    double num = (q1.x - p1.x) * sy - (q1.y - p1.y) * sx;
+    double t = num / den;
+    double den2 = den * den;
+    /* dt/d(p1) */
+    d2 tp1 = mk(((-sy) * den - num * (-sy)) / den2, ((sx) * den - num * (sx)) / den2);
+    /* dt/d(p2): numerator does not depend on p2 */
+    d2 tp2 = mk((-num * sy) / den2, (num * sx) / den2);
+    /* dt/d(q1): s = q2 - q1 depends on q1 */
+    double nq1x = sy + (q1.y - p1.y), nq1y = -sx - (q1.x - p1.x);
+    d2 tq1 = mk((nq1x * den - num * ry) / den2, (nq1y * den - num * (-rx)) / den2);
+    double nq2x = -(q1.y - p1.y), nq2y = (q1.x - p1.x);
+    d2 tq2 = mk((nq2x * den - num * (-ry)) / den2, (nq2y * den - num * rx) / den2);
+    /* out = p1 + t r */
+    gp1->x = (1.0 - t + rx * tp1.x) * g.x + (ry * tp1.x) * g.y;
+    gp1->y = (rx * tp1.y) * g.x + (1.0 - t + ry * tp1.y) * g.y;
+    gp2->x = (t + rx * tp2.x) * g.x + (ry * tp2.x) * g.y;
+    gp2->y = (rx * tp2.y) * g.x + (t + ry * tp2.y) * g.y;
+    gq1->x = (rx * tq1.x) * g.x + (ry * tq1.x) * g.y;
+    gq1->y = (rx * tq1.y) * g.x + (ry * tq1.y) * g.y;
+    gq2->x = (rx * tq2.x) * g.x + (ry * tq2.x) * g.y;
+    gq2->y = (rx * tq2.y) * g.x + (ry * tq2.y) * g.y;
+}
+
+/* Per-stage record of where each clipped vertex came from
+ * (core/pack_cuda_primitives.py:228-235). */
+typedef struct {
+    int n;
+    int kind[OR_MAXI];   /* 0: input vertex, 1: edge/line intersection */
+    int a[OR_MAXI];      /* input index (kind 0) or edge start (kind 1) */
+    int b[OR_MAXI];      /* edge end (kind 1) */
+} clip_rec;
+
+/* One Sutherland-Hodgman stage against edge A->B, inside = cross >= 0
+ * (core/pack_cuda_primitives.py:238-295). */
+static int clip_stage(const d2 *in, int n, d2 A, d2 B, d2 *out, clip_rec *r) {
+    int cnt = 0, i;
+    if (n == 0) { r->n = 0; return 0; }
+    int si = n - 1;
+    d2 S = in[si];
+    double ss = cross3(A, B, S);
+    for (i = 0; i < n; ++i) {
+        d2 E = in[i];
+        double es = cross3(A, B, E);
+        int sin_ = ss >= 0.0, ein = es >= 0.0;
+        if (sin_ && ein) {
+            out[cnt] = E; r->kind[cnt] = 0; r->a[cnt] = i; cnt++;
+        } else if (sin_ && !ein) {
+            out[cnt] = line_isect(S, E, A, B); r->kind[cnt] = 1; r->a[cnt] = si; r->b[cnt] = i; cnt++;
+        } else if (!sin_ && ein) {
+            out[cnt] = line_isect(S, E, A, B); r->kind[cnt] = 1; r->a[cnt] = si; r->b[cnt] = i; cnt++;
+            out[cnt] = E; r->kind[cnt] = 0; r->a[cnt] = i; cnt++;
+        }
+        S = E; ss = es; si = i;
+    }
+    r->n = cnt;
+    return cnt;
+}
+
+/* Reverse of one stage (core/pack_cuda_primitives.py:299-342). */
+static void clip_stage_bwd(const d2 *in, int n, d2 A, d2 B, const d2 *gout,
+                           const clip_rec *r, d2 *gin, d2 *gA, d2 *gB) {
+    int i;
+    for (i = 0; i < n; ++i) gin[i] = mk(0, 0);
+    *gA = mk(0, 0); *gB = mk(0, 0);
+    for (i = 0; i < r->n; ++i) {
+        if (r->kind[i] == 0) {
+            gin[r->a[i]].x += gout[i].x;
+            gin[r->a[i]].y += gout[i].y;
+        } else {
+            d2 gS, gE, ga, gb;
+            line_isect_bwd(in[r->a[i]], in[r->b[i]], A, B, gout[i], &gS, &gE, &ga, &gb);
+            gin[r->a[i]].x += gS.x; gin[r->a[i]].y += gS.y;
+            gin[r->b[i]].x += gE.x; gin[r->b[i]].y += gE.y;
+            gA->x += ga.x; gA->y += ga.y;
+            gB->x += gb.x; gB->y += gb.y;
+        }
+    }
+}
+
+/* Area of subj ∩ clip by successive clipping of subj, with gradients w.r.t. both
+ * vertex lists when dsubj/dclip are non-NULL (core/pack_cuda_primitives.py:348-538). */
+double or_clip_area(const double *subj_, int ns, const double *clip_, int nc,
+                    double *dsubj_, double *dclip_) {
+    d2 subj[OR_MAXV], clip[OR_MAXV];
+    d2 polys[OR_MAXV + 1][OR_MAXI];
+    int counts[OR_MAXV + 1];
+    clip_rec rec[OR_MAXV];
+    int i, e;
+    int grads = dsubj_ != NULL && dclip_ != NULL;
+    for (i = 0; i < ns; ++i) subj[i] = mk(subj_[2 * i], subj_[2 * i + 1]);
+    for (i = 0; i < nc; ++i) clip[i] = mk(clip_[2 * i], clip_[2 * i + 1]);
+    if (grads) {
+        for (i = 0; i < 2 * ns; ++i) dsubj_[i] = 0.0;
+        for (i = 0; i < 2 * nc; ++i) dclip_[i] = 0.0;
+    }
+    if (ns == 0 || nc == 0) return 0.0;
+    counts[0] = ns;
+    for (i = 0; i < ns; ++i) polys[0][i] = subj[i];
+    int stages = 0;
+    for (e = 0; e < nc && counts[e] > 0; ++e) {
+        counts[e + 1] = clip_stage(polys[e], counts[e], clip[e], clip[(e + 1) % nc],
+                                   polys[e + 1], &rec[e]);
+        stages++;
+    }
+    int fn = counts[stages];
+    if (fn < 3) return 0.0;
+    d2 gfin[OR_MAXI];
+    double area = poly_area_grad(polys[stages], fn, gfin);
+    if (!grads || area == 0.0) return area;
+    d2 gcur[OR_MAXI];
+    for (i = 0; i < fn; ++i) gcur[i] = gfin[i];
+    for (e = stages - 1; e >= 0; --e) {
+        int ea = e % nc, eb = (e + 1) % nc;
+        d2 gin[OR_MAXI], gA, gB;
+        clip_stage_bwd(polys[e], counts[e], clip[ea], clip[eb], gcur, &rec[e], gin, &gA, &gB);
+        dclip_[2 * ea] += gA.x; dclip_[2 * ea + 1] += gA.y;
+        dclip_[2 * eb] += gB.x; dclip_[2 * eb + 1] += gB.y;
+        for (i = 0; i < counts[e]; ++i) gcur[i] = gin[i];
+    }
+    for (i = 0; i < ns; ++i) { dsubj_[2 * i] = gcur[i].x; dsubj_[2 * i + 1] = gcur[i].y; }
+    return area;
+}
+
+/* SAT penetration depth of piece 1 (local verts + pose) into piece 2 (world verts) and
+ * its gradient w.r.t. pose 1 (core/pack_cuda_primitives.py:544-828). */
+void or_sat(const double *v1l, int n1, const double *v2w_, int n2, double x1, double y1,
+            double c1, double s1, double *sep, double *gx, double *gy, double *gt) {
+    d2 w1[OR_MAXV], v2w[OR_MAXV];
+    double p1[OR_MAXV], p2[OR_MAXV];
+    int i, k, pass;
+    int grads = gx != NULL && gy != NULL && gt != NULL;
+    *sep = 0.0;
+    if (grads) { *gx = 0.0; *gy = 0.0; *gt = 0.0; }
+    if (n1 < 3 || n2 < 3 || n1 > OR_MAXV || n2 > OR_MAXV) return;
+    for (i = 0; i < n1; ++i) {
+        double lx = v1l[2 * i], ly = v1l[2 * i + 1];
+        w1[i] = mk(c1 * lx - s1 * ly + x1, s1 * lx + c1 * ly + y1);
+    }
+    for (i = 0; i < n2; ++i) v2w[i] = mk(v2w_[2 * i], v2w_[2 * i + 1]);
+    double best = 1.0e30;
+    int found = 0, bcase = -1, bv = -1, from1 = 0;
+    d2 bn = mk(0, 0), bw = mk(0, 0);
+    for (pass = 0; pass < 2; ++pass) {
+        const d2 *ev = pass == 0 ? w1 : v2w;
+        int ne = pass == 0 ? n1 : n2;
+        for (i = 0; i < ne; ++i) {
+            d2 a = ev[i], b = ev[(i + 1) % ne];
+            d2 nrm = mk(-(b.y - a.y), b.x - a.x);
+            double len = sqrt(nrm.x * nrm.x + nrm.y * nrm.y);
+            if (len < 1.0e-12) continue;
+            nrm.x /= len; nrm.y /= len;
+            for (k = 0; k < n1; ++k) p1[k] = w1[k].x * nrm.x + w1[k].y * nrm.y;
+            for (k = 0; k < n2; ++k) p2[k] = v2w[k].x * nrm.x + v2w[k].y * nrm.y;
+            double mn1 = p1[0], mx1 = p1[0], mn2 = p2[0], mx2 = p2[0];
+            for (k = 1; k < n1; ++k) { if (p1[k] < mn1) mn1 = p1[k]; if (p1[k] > mx1) mx1 = p1[k]; }
+            for (k = 1; k < n2; ++k) { if (p2[k] < mn2) mn2 = p2[k]; if (p2[k] > mx2) mx2 = p2[k]; }
+            if (mx1 < mn2 || mx2 < mn1) return;            /* separating axis */
+            found = 1;
+            double pen1 = mx1 - mn2, pen2 = mx2 - mn1;
+            double pen = pen1 < pen2 ? pen1 : pen2;
+            if (pen < best) {
+                best = pen; bn = nrm; from1 = pass == 0;
+                int iv = 0, iw = 0;
+                if (pen1 <= pen2) {
+                    bcase = 1;
+                    for (k = 1; k < n1; ++k) if (p1[k] > p1[iv]) iv = k;
+                    for (k = 1; k < n2; ++k) if (p2[k] < p2[iw]) iw = k;
+                } else {
+                    bcase = 2;
+                    for (k = 1; k < n1; ++k) if (p1[k] < p1[iv]) iv = k;
+                    for (k = 1; k < n2; ++k) if (p2[k] > p2[iw]) iw = k;
+                }
+                bv = iv; bw = v2w[iw];
+            }
+        }
+    }
+    if (!found || best <= 1.0e-12) return;
+    *sep = best;
+    if (!grads) return;
+    d2 gv = bcase == 1 ? bn : mk(-bn.x, -bn.y);
+    double vx0 = v1l[2 * bv], vy0 = v1l[2 * bv + 1];
+    double dvx = -s1 * vx0 - c1 * vy0, dvy = c1 * vx0 - s1 * vy0;
+    double tv = gv.x * dvx + gv.y * dvy;
+    double ta = 0.0;
+    if (from1) {
+        d2 vw = w1[bv];
+        d2 gn = bcase == 1 ? mk(vw.x - bw.x, vw.y - bw.y) : mk(bw.x - vw.x, bw.y - vw.y);
+        ta = gn.x * (-bn.y) + gn.y * bn.x;
+    }
+    *gx = gv.x; *gy = gv.y; *gt = tv + ta;
+}
+
+/* ---------------------------------------------------------------------------
+ * Exact pair loop: core/pack_cuda.py:95-546
+ * ------------------------------------------------------------------------- */
+static void piece_world(const or_geom *g, double px, double py, double c, double s, int pi,
+                        d2 *out, double *bb) {
+    int n = g->nverts[pi], v;
+    double mnx = 1e30, mxx = -1e30, mny = 1e30, mxy = -1e30;
+    for (v = 0; v < n; ++v) {
+        double x = g->piece_xy[(pi * g->V + v) * 2], y = g->piece_xy[(pi * g->V + v) * 2 + 1];
+        double xt = c * x - s * y + px, yt = s * x + c * y + py;
+        out[v] = mk(xt, yt);
+        mnx = fmin(mnx, xt); mxx = fmax(mxx, xt); mny = fmin(mny, yt); mxy = fmax(mxy, yt);
+    }
+    bb[0] = mnx; bb[1] = mxx; bb[2] = mny; bb[3] = mxy;
+}
+
+/* One ref piece against all pieces of one other tree (core/pack_cuda.py:142-252). */
+static double pair_piece(const or_geom *g, const double *ref, const double *oth, int pi,
+                         const d2 *rpoly, const double *rbb, double cr, double sr,
+                         int use_sep, int grads, double *dref) {
+    double dx = oth[0] - ref[0], dy = oth[1] - ref[1];
+    if (dx * dx + dy * dy > g->cutoff_sq) return 0.0;
+    double co = cos(oth[2]), so = sin(oth[2]);
+    int n1 = g->nverts[pi], pj, v;
+    double tot = 0.0;
+    double rl[2 * OR_MAXV];
+    for (v = 0; v < n1; ++v) {
+        rl[2 * v] = g->piece_xy[(pi * g->V + v) * 2];
+        rl[2 * v + 1] = g->piece_xy[(pi * g->V + v) * 2 + 1];
+    }
+    for (pj = 0; pj < g->P; ++pj) {
+        d2 op[OR_MAXV];
+        double obb[4];
+        int n2 = g->nverts[pj];
+        piece_world(g, oth[0], oth[1], co, so, pj, op, obb);
+        if (rbb[1] < obb[0] || obb[1] < rbb[0] || rbb[3] < obb[2] || obb[3] < rbb[2]) continue;
+        if (!use_sep) {
+            double ds[2 * OR_MAXV], dc[2 * OR_MAXV];
+            double a = or_clip_area((const double *)rpoly, n1, (const double *)op, n2,
+                                    grads ? ds : NULL, grads ? dc : NULL);
+            tot += a;
+            if (grads) {
+                /* core/pack_cuda_primitives.py:200-225 */
+                double gx = 0, gy = 0, gt = 0;
+                for (v = 0; v < n1; ++v) {
+                    gx += ds[2 * v];
+                    gy += ds[2 * v + 1];
+                    gt += ds[2 * v] * (-sr * rl[2 * v] - cr * rl[2 * v + 1]);
+                    gt += ds[2 * v + 1] * (cr * rl[2 * v] - sr * rl[2 * v + 1]);
+                }
+                dref[0] += gx; dref[1] += gy; dref[2] += gt;
+            }
+        } else {
+            double sp, gx, gy, gt;
+            or_sat(rl, n1, (const double *)op, n2, ref[0], ref[1], cr, sr, &sp,
+                   grads ? &gx : NULL, grads ? &gy : NULL, grads ? &gt : NULL);
+            if (sp > 0.0) {
+                tot += sp * sp;
+                if (grads) {
+                    dref[0] += 2.0 * sp * gx; dref[1] += 2.0 * sp * gy; dref[2] += 2.0 * sp * gt;
+                }
+            }
+        }
+    }
+    return tot;
+}
+
+/* overlap_multi_ensemble: ordered pairs, self skipped, halved total, gradient to the
+ * reference tree of each ordered pair (core/pack_cuda.py:254-546). */
+void or_overlap(const or_geom *g, const double *xyt, int n_ens, int n_trees, int use_sep,
+                const double *crystal, int only_self, int n_periodic,
+                double *out_cost, double *out_grads) {
+    int e;
+    for (e = 0; e < n_ens; ++e) {
+        const double *X = xyt + (size_t)e * n_trees * 3;
+        double *G = out_grads ? out_grads + (size_t)e * n_trees * 3 : NULL;
+        double total = 0.0;
+        int i, j, pi, cx, cy;
+        double ax = 0, ay = 0, bx = 0, by = 0;
+        if (crystal) { ax = crystal[e * 4]; ay = crystal[e * 4 + 1]; bx = crystal[e * 4 + 2]; by = crystal[e * 4 + 3]; }
+        if (G) memset(G, 0, sizeof(double) * n_trees * 3);
+        for (i = 0; i < n_trees; ++i) {
+            const double *ref = X + 3 * i;
+            double cr = cos(ref[2]), sr = sin(ref[2]);
+            double local = 0.0, d[3] = {0, 0, 0};
+            for (pi = 0; pi < g->P; ++pi) {
+                d2 rp[OR_MAXV];
+                double rbb[4];
+                piece_world(g, ref[0], ref[1], cr, sr, pi, rp, rbb);
+                for (j = 0; j < n_trees; ++j) {
+                    const double *ob = X + 3 * j;
+                    int self = i == j;
+                    if (only_self && !self) continue;
+                    if (!crystal) {
+                        double dx = ob[0] - ref[0], dy = ob[1] - ref[1];
+                        if (dx * dx + dy * dy > g->cutoff_sq) continue;
+                        if (self) continue;
+                        local += pair_piece(g, ref, ob, pi, rp, rbb, cr, sr, use_sep, G != NULL, d);
+                    } else {
+                        for (cx = -n_periodic; cx <= n_periodic; ++cx)
+                            for (cy = -n_periodic; cy <= n_periodic; ++cy) {
+                                if (self && cx == 0 && cy == 0) continue;
+                                double o[3];
+                                o[0] = ob[0] + cx * ax + cy * bx;
+                                o[1] = ob[1] + cx * ay + cy * by;
+                                o[2] = ob[2];
+                                local += pair_piece(g, ref, o, pi, rp, rbb, cr, sr, use_sep,
+                                                    self ? 0 : G != NULL, d);
+                            }
+                    }
+                }
+            }
+            if (G) { G[3 * i] += d[0]; G[3 * i + 1] += d[1]; G[3 * i + 2] += d[2]; }
+            total += only_self ? local : local / 2.0;
+        }
+        out_cost[e] = total;
+    }
+}
+
+/* Boundary term: per tree the max over outline vertices of the squared distance outside
+ * the square h = [size, ox, oy] (core/pack_cuda.py:550-757). */
+void or_boundary(const or_geom *g, const double *xyt, const double *h, int n_ens, int n_trees,
+                 double *out_cost, double *out_grads, double *out_grad_h) {
+    int e, t, v;
+    for (e = 0; e < n_ens; ++e) {
+        double half = h[3 * e] * 0.5, ox = h[3 * e + 1], oy = h[3 * e + 2];
+        double total = 0.0, gh[3] = {0, 0, 0};
+        for (t = 0; t < n_trees; ++t) {
+            const double *p = xyt + ((size_t)e * n_trees + t) * 3;
+            double c = cos(p[2]), s = sin(p[2]);
+            double best = 0.0;
+            int bi = 0;
+            for (v = 0; v < g->n_outline; ++v) {
+                double lx = g->outline[2 * v], ly = g->outline[2 * v + 1];
+                double vx = c * lx - s * ly + p[0] - ox, vy = s * lx + c * ly + p[1] - oy;
+                double ax_ = fabs(vx), ay_ = fabs(vy);
+                double dx = ax_ > half ? ax_ - half : 0.0, dy = ay_ > half ? ay_ - half : 0.0;
+                double d2_ = dx * dx + dy * dy;
+                if (d2_ > best) { best = d2_; bi = v; }
+            }
+            total += best;
+            double lx = g->outline[2 * bi], ly = g->outline[2 * bi + 1];
+            double vx = c * lx - s * ly + p[0] - ox, vy = s * lx + c * ly + p[1] - oy;
+            double ax_ = fabs(vx), ay_ = fabs(vy);
+            double dx = ax_ > half ? ax_ - half : 0.0, dy = ay_ > half ? ay_ - half : 0.0;
+            double gvx = dx > 0.0 ? 2.0 * dx * (vx >= 0.0 ? 1.0 : -1.0) : 0.0;
+            double gvy = dy > 0.0 ? 2.0 * dy * (vy >= 0.0 ? 1.0 : -1.0) : 0.0;
+            if (out_grads) {
+                double *G = out_grads + ((size_t)e * n_trees + t) * 3;
+                G[0] = gvx; G[1] = gvy;
+                G[2] = gvx * (-s * lx - c * ly) + gvy * (c * lx - s * ly);
+            }
+            gh[0] += -(dx + dy); gh[1] += -gvx; gh[2] += -gvy;
+        }
+        out_cost[e] = total;
+        if (out_grad_h) { out_grad_h[3 * e] = gh[0]; out_grad_h[3 * e + 1] = gh[1]; out_grad_h[3 * e + 2] = gh[2]; }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * LUT: core/pack_cuda_lut.py:289-640
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny, nt;
+    double x_min, x_max, y_min, y_max, t_min, t_max;
+    int quad;      /* max(0, v)^2 per pair */
+    int mirror;    /* table holds x >= 0 only; x < 0 looked up as (-x, -theta) */
+    const float *vals;   /* nx*ny*nt, theta fastest */
+} or_lut;
+
+static double wrap_pi(double t) {
+    while (t > M_PI) t -= 2.0 * M_PI;
+    while (t < -M_PI) t += 2.0 * M_PI;
+    return t;
+}
+
+/* Trilinear value and gradient (core/pack_cuda_lut.py:298-415). */
+double or_lut_lookup(const or_lut *L, double x, double y, double th, double *g) {
+    double xlo = L->mirror ? -L->x_max : L->x_min;
+    if (x < xlo || x > L->x_max || y < L->y_min || y > L->y_max || th < L->t_min || th > L->t_max) {
+        if (g) { g[0] = g[1] = g[2] = 0.0; }
+        return 0.0;
+    }
+    double xt = x, tt = th;
+    int flip = L->mirror && x < 0.0;
+    if (flip) { xt = -x; tt = -th; }
+    double gx = (xt - L->x_min) / (L->x_max - L->x_min) * (L->nx - 1);
+    double gy = (y - L->y_min) / (L->y_max - L->y_min) * (L->ny - 1);
+    double gt = (tt - L->t_min) / (L->t_max - L->t_min) * (L->nt - 1);
+    int ix0 = (int)floor(gx), iy0 = (int)floor(gy), it0 = (int)floor(gt);
+    int ix1 = ix0 + 1 < L->nx - 1 ? ix0 + 1 : L->nx - 1;
+    int iy1 = iy0 + 1 < L->ny - 1 ? iy0 + 1 : L->ny - 1;
+    int it1 = it0 + 1 < L->nt - 1 ? it0 + 1 : L->nt - 1;
+    double fx = gx - ix0, fy = gy - iy0, ft = gt - it0;
+#define LV(a, b, c) ((double)L->vals[((size_t)(a) * L->ny + (b)) * L->nt + (c)])
+    double v000 = LV(ix0, iy0, it0), v001 = LV(ix0, iy0, it1);
+    double v010 = LV(ix0, iy1, it0), v011 = LV(ix0, iy1, it1);
+    double v100 = LV(ix1, iy0, it0), v101 = LV(ix1, iy0, it1);
+    double v110 = LV(ix1, iy1, it0), v111 = LV(ix1, iy1, it1);
+#undef LV
+    double v00 = v000 * (1.0 - ft) + v001 * ft, v01 = v010 * (1.0 - ft) + v011 * ft;
+    double v10 = v100 * (1.0 - ft) + v101 * ft, v11 = v110 * (1.0 - ft) + v111 * ft;
+    double v0 = v00 * (1.0 - fy) + v01 * fy, v1 = v10 * (1.0 - fy) + v11 * fy;
+    double val = v0 * (1.0 - fx) + v1 * fx;
+    if (g) {
+        double dfx = v1 - v0;
+        double dfy = (1.0 - fx) * (v01 - v00) + fx * (v11 - v10);
+        double d0 = (v001 - v000) * (1.0 - fy) + (v011 - v010) * fy;
+        double d1 = (v101 - v100) * (1.0 - fy) + (v111 - v110) * fy;
+        double dft = d0 * (1.0 - fx) + d1 * fx;
+        g[0] = dfx * ((double)(L->nx - 1) / (L->x_max - L->x_min));
+        g[1] = dfy * ((double)(L->ny - 1) / (L->y_max - L->y_min));
+        g[2] = dft * ((double)(L->nt - 1) / (L->t_max - L->t_min));
+        if (flip) { g[0] = -g[0]; g[2] = -g[2]; }
+    }
+    return val;
+}
+
+/* i<j pairs, relative pose in the ref frame, gradient to both trees, total not halved
+ * (core/pack_cuda_lut.py:420-640). */
+void or_lut_overlap(const or_lut *L, double radius, const double *xyt, int n_ens, int n_trees,
+                    double *out_cost, double *out_grads) {
+    int e, i, j;
+    double cut = 4.0 * radius * radius;
+    for (e = 0; e < n_ens; ++e) {
+        const double *X = xyt + (size_t)e * n_trees * 3;
+        double *G = out_grads ? out_grads + (size_t)e * n_trees * 3 : NULL;
+        double total = 0.0;
+        if (G) memset(G, 0, sizeof(double) * n_trees * 3);
+        for (i = 0; i < n_trees; ++i) {
+            double cr = cos(X[3 * i + 2]), sr = sin(X[3 * i + 2]);
+            double sum = 0.0;
+            for (j = i + 1; j < n_trees; ++j) {
+                double dxw = X[3 * j] - X[3 * i], dyw = X[3 * j + 1] - X[3 * i + 1];
+                if (dxw * dxw + dyw * dyw > cut) continue;
+                double dxl = cr * dxw + sr * dyw, dyl = -sr * dxw + cr * dyw;
+                double dth = wrap_pi(X[3 * j + 2] - X[3 * i + 2]);
+                double gl[3];
+                double v = or_lut_lookup(L, dxl, dyl, dth, G ? gl : NULL);
+                if (L->quad) {
+                    if (v > 0.0) {
+                        if (G) { gl[0] *= 2.0 * v; gl[1] *= 2.0 * v; gl[2] *= 2.0 * v; }
+                        v = v * v;
+                    } else {
+                        v = 0.0;
+                        if (G) gl[0] = gl[1] = gl[2] = 0.0;
+                    }
+                }
+                sum += v;
+                if (G) {
+                    G[3 * i] += -gl[0] * cr + gl[1] * sr;
+                    G[3 * i + 1] += -gl[0] * sr - gl[1] * cr;
+                    G[3 * i + 2] += gl[0] * dyl - gl[1] * dxl - gl[2];
+                    G[3 * j] += gl[0] * cr - gl[1] * sr;
+                    G[3 * j + 1] += gl[0] * sr + gl[1] * cr;
+                    G[3 * j + 2] += gl[2];
+                }
+            }
+            total += sum;
+        }
+        out_cost[e] = total;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Hungarian: core/lap_batch.py:62-196.  float32 arithmetic as in the reference, and an
+ * int64 restatement of the same scan order for integer matrices.
+ * ------------------------------------------------------------------------- */
+void or_hungarian_f32(const float *cost, int B, int N, int *row_match, float *total) {
+    int b, i, j;
+    float *u = (float *)malloc(sizeof(float) * N), *v = (float *)malloc(sizeof(float) * N);
+    float *minv = (float *)malloc(sizeof(float) * N);
+    int *p = (int *)malloc(sizeof(int) * N), *cm = (int *)malloc(sizeof(int) * N);
+    char *used = (char *)malloc(N);
+    for (b = 0; b < B; ++b) {
+        const float *C = cost + (size_t)b * N * N;
+        int *rm = row_match + (size_t)b * N;
+        for (i = 0; i < N; ++i) { rm[i] = -1; cm[i] = -1; u[i] = 0.0f; v[i] = 0.0f; }
+        for (i = 0; i < N; ++i) {
+            for (j = 0; j < N; ++j) { minv[j] = 1e30f; used[j] = 0; p[j] = -1; }
+            int cr = i, cc = -1;
+            for (;;) {
+                int bc = -1;
+                float bv = 1e30f;
+                for (j = 0; j < N; ++j) {
+                    if (used[j]) continue;
+                    float red = C[(size_t)cr * N + j] - u[cr] - v[j];
+                    if (red < minv[j]) { minv[j] = red; p[j] = cr; }
+                    if (minv[j] < bv) { bv = minv[j]; bc = j; }
+                }
+                for (j = 0; j < N; ++j) {
+                    if (used[j]) { u[cm[j]] += bv; v[j] -= bv; }
+                    else minv[j] -= bv;
+                }
+                u[i] += bv;
+                used[bc] = 1;
+                cc = bc;
+                cr = cm[cc];
+                if (cr < 0) break;
+            }
+            while (cc >= 0) {
+                int pr = p[cc], pc = rm[pr];
+                rm[pr] = cc; cm[cc] = pr; cc = pc;
+            }
+        }
+        float t = 0.0f;
+        for (i = 0; i < N; ++i) { int c = rm[i]; if (c >= 0 && c < N) t += C[(size_t)i * N + c]; }
+        total[b] = t;
+    }
+    free(u); free(v); free(minv); free(p); free(cm); free(used);
+}
+
+void or_hungarian_i32(const int32_t *cost, int B, int N, int *row_match, int64_t *total) {
+    int b, i, j;
+    int64_t *u = (int64_t *)malloc(sizeof(int64_t) * N), *v = (int64_t *)malloc(sizeof(int64_t) * N);
+    int64_t *minv = (int64_t *)malloc(sizeof(int64_t) * N);
+    int *p = (int *)malloc(sizeof(int) * N), *cm = (int *)malloc(sizeof(int) * N);
+    char *used = (char *)malloc(N);
+    const int64_t BIG = (int64_t)1 << 62;
+    for (b = 0; b < B; ++b) {
+        const int32_t *C = cost + (size_t)b * N * N;
+        int *rm = row_match + (size_t)b * N;
+        for (i = 0; i < N; ++i) { rm[i] = -1; cm[i] = -1; u[i] = 0; v[i] = 0; }
+        for (i = 0; i < N; ++i) {
+            for (j = 0; j < N; ++j) { minv[j] = BIG; used[j] = 0; p[j] = -1; }
+            int cr = i, cc = -1;
+            for (;;) {
+                int bc = -1;
+                int64_t bv = BIG;
+                for (j = 0; j < N; ++j) {
+                    if (used[j]) continue;
+                    int64_t red = (int64_t)C[(size_t)cr * N + j] - u[cr] - v[j];
+                    if (red < minv[j]) { minv[j] = red; p[j] = cr; }
+                    if (minv[j] < bv) { bv = minv[j]; bc = j; }
+                }
+                for (j = 0; j < N; ++j) {
+                    if (used[j]) { u[cm[j]] += bv; v[j] -= bv; }
+                    else minv[j] -= bv;
+                }
+                u[i] += bv;
+                used[bc] = 1;
+                cc = bc;
+                cr = cm[cc];
+                if (cr < 0) break;
+            }
+            while (cc >= 0) {
+                int pr = p[cc], pc = rm[pr];
+                rm[pr] = cc; cm[cc] = pr; cc = pc;
+            }
+        }
+        int64_t t = 0;
+        for (i = 0; i < N; ++i) t += C[(size_t)i * N + rm[i]];
+        total[b] = t;
+    }
+    free(u); free(v); free(minv); free(p); free(cm); free(used);
+}
+
+/* Diversity shortcut in float32 (core/lap_batch.py:216-331). */
+void or_diversity(const float *pop, const float *ref, int n_pop, int n_ref, int n_trees,
+                  float cos_a, float sin_a, int mirror, float *out) {
+    const float PI = 3.14159265358979323846f, TWO_PI = 2.0f * PI;
+    float rot = atan2f(sin_a, cos_a);
+    float *rx = (float *)malloc(sizeof(float) * n_trees * 3);
+    int a, b, t, j;
+    for (a = 0; a < n_pop; ++a)
+        for (b = 0; b < n_ref; ++b) {
+            const float *P = pop + (size_t)a * n_trees * 3, *R = ref + (size_t)b * n_trees * 3;
+            for (t = 0; t < n_trees; ++t) {
+                float x = R[3 * t], y = R[3 * t + 1], th = R[3 * t + 2];
+                if (mirror) { y = -y; th = PI - th; }
+                float xr = x * cos_a - y * sin_a, yr = x * sin_a + y * cos_a, tr = th + rot;
+                tr = fmodf(tr + PI, TWO_PI);
+                if (tr < 0) tr += TWO_PI;
+                tr -= PI;
+                rx[3 * t] = xr; rx[3 * t + 1] = yr; rx[3 * t + 2] = tr;
+            }
+            float tot = 0.0f;
+            for (t = 0; t < n_trees; ++t) {
+                float best = 1e30f;
+                for (j = 0; j < n_trees; ++j) {
+                    float dx = P[3 * t] - rx[3 * j], dy = P[3 * t + 1] - rx[3 * j + 1];
+                    float dt = P[3 * t + 2] - rx[3 * j + 2];
+                    dt = fmodf(dt + PI, TWO_PI);
+                    if (dt < 0) dt += TWO_PI;
+                    dt -= PI;
+                    float sq = dx * dx + dy * dy + dt * dt;
+                    if (sq < best) best = sq;
+                }
+                tot += sqrtf(best);
+            }
+            out[(size_t)a * n_ref + b] = tot;
+        }
+    free(rx);
+}
+
+/* ---------------------------------------------------------------------------
+ * Feasibility: trees overlap iff some piece pair has intersecting interiors, i.e. no
+ * edge normal separates them with max1 <= min2 or max2 <= min1 (touching allowed).
+ * Semantics of core/pack_metric.py:130-141 ("intersects and not touches").
+ * ------------------------------------------------------------------------- */
+static int pieces_interiors_meet(const d2 *a, int na, const d2 *b, int nb) {
+    int pass, i, k;
+    for (pass = 0; pass < 2; ++pass) {
+        const d2 *e = pass == 0 ? a : b;
+        int ne = pass == 0 ? na : nb;
+        for (i = 0; i < ne; ++i) {
+            d2 p = e[i], q = e[(i + 1) % ne];
+            double nx = -(q.y - p.y), ny = q.x - p.x;
+            if (nx == 0.0 && ny == 0.0) continue;
+            double mn1 = 1e300, mx1 = -1e300, mn2 = 1e300, mx2 = -1e300;
+            for (k = 0; k < na; ++k) { double d = a[k].x * nx + a[k].y * ny; if (d < mn1) mn1 = d; if (d > mx1) mx1 = d; }
+            for (k = 0; k < nb; ++k) { double d = b[k].x * nx + b[k].y * ny; if (d < mn2) mn2 = d; if (d > mx2) mx2 = d; }
+            if (mx1 <= mn2 || mx2 <= mn1) return 0;
+        }
+    }
+    return 1;
+}
+
+/* out[e] = number of tree pairs whose interiors intersect. */
+void or_feasible(const or_geom *g, const double *xyt, int n_ens, int n_trees, int *out_pairs) {
+    int e, i, j, pi, pj;
+    for (e = 0; e < n_ens; ++e) {
+        const double *X = xyt + (size_t)e * n_trees * 3;
+        int cnt = 0;
+        for (i = 0; i < n_trees; ++i)
+            for (j = i + 1; j < n_trees; ++j) {
+                double dx = X[3 * j] - X[3 * i], dy = X[3 * j + 1] - X[3 * i + 1];
+                if (dx * dx + dy * dy > g->cutoff_sq) continue;
+                double ci = cos(X[3 * i + 2]), si = sin(X[3 * i + 2]);
+                double cj = cos(X[3 * j + 2]), sj = sin(X[3 * j + 2]);
+                int hit = 0;
+                for (pi = 0; pi < g->P && !hit; ++pi) {
+                    d2 a[OR_MAXV];
+                    double bb[4];
+                    piece_world(g, X[3 * i], X[3 * i + 1], ci, si, pi, a, bb);
+                    for (pj = 0; pj < g->P && !hit; ++pj) {
+                        d2 b[OR_MAXV];
+                        double bb2[4];
+                        piece_world(g, X[3 * j], X[3 * j + 1], cj, sj, pj, b, bb2);
+                        hit = pieces_interiors_meet(a, g->nverts[pi], b, g->nverts[pj]);
+                    }
+                }
+                cnt += hit;
+            }
+        out_pairs[e] = cnt;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Batched L-BFGS without line search: core/lbfgs_torch_parallel.py:411-729.
+ * M systems of n parameters; f(ctx, x, M, n, loss, grad).  The reference's masked
+ * two-loop, step rule, max-step clamp, history append/shift and convergence tests are
+ * restated; inactive systems keep stepping as in the reference.
+ * ------------------------------------------------------------------------- */
+typedef void (*or_fn)(void *ctx, const double *x, int M, int n, double *loss, double *grad);
+
+static double vdot(const double *a, const double *b, int n) {
+    double s = 0.0;
+    int k;
+    for (k = 0; k < n; ++k) s += a[k] * b[k];
+    return s;
+}
+
+int or_lbfgs(or_fn f, void *ctx, double *x0, int M, int n, double lr, int max_iter, int max_eval,
+             double tol_grad, double tol_change, double tol_rel, int stop_on_increase, int m,
+             double max_step /* <= 0: none */) {
+    size_t Mn = (size_t)M * n;
+    double *x = (double *)malloc(sizeof(double) * Mn), *g = (double *)malloc(sizeof(double) * Mn);
+    double *gp = (double *)malloc(sizeof(double) * Mn), *d = (double *)malloc(sizeof(double) * Mn);
+    double *q = (double *)malloc(sizeof(double) * n), *y = (double *)malloc(sizeof(double) * n);
+    double *sv = (double *)malloc(sizeof(double) * n);
+    double *Y = (double *)calloc((size_t)M * m * n, sizeof(double));
+    double *S = (double *)calloc((size_t)M * m * n, sizeof(double));
+    double *ro = (double *)calloc((size_t)M * m, sizeof(double)), *al = (double *)malloc(sizeof(double) * m);
+    double *loss = (double *)malloc(sizeof(double) * M), *lprev = (double *)malloc(sizeof(double) * M);
+    double *l0 = (double *)malloc(sizeof(double) * M), *H = (double *)malloc(sizeof(double) * M);
+    double *t = (double *)malloc(sizeof(double) * M);
+    int *hl = (int *)calloc(M, sizeof(int)), *act = (int *)malloc(sizeof(int) * M);
+    long *fe = (long *)malloc(sizeof(long) * M);
+    int a, k, i, it = 0;
+    if (max_eval <= 0) max_eval = max_iter * 5 / 4;
+    memcpy(x, x0, sizeof(double) * Mn);
+    f(ctx, x, M, n, loss, g);
+    int any = 0;
+    for (a = 0; a < M; ++a) {
+        double gn = 0.0;
+        for (k = 0; k < n; ++k) gn = fmax(gn, fabs(g[(size_t)a * n + k]));
+        act[a] = !(gn <= tol_grad);
+        any |= act[a];
+        fe[a] = 1; H[a] = 1.0; l0[a] = loss[a];
+    }
+    while (any && it < max_iter) {
+        it++;
+        for (a = 0; a < M; ++a) {
+            double *ga = g + (size_t)a * n, *da = d + (size_t)a * n;
+            if (it == 1) {
+                for (k = 0; k < n; ++k) da[k] = -ga[k];
+                H[a] = 1.0;
+            } else {
+                double *Ya = Y + (size_t)a * m * n, *Sa = S + (size_t)a * m * n;
+                for (k = 0; k < n; ++k) q[k] = -ga[k];
+                for (i = 0; i < m; ++i) al[i] = 0.0;
+                for (i = m - 1; i >= 0; --i) {
+                    if (i >= hl[a]) continue;
+                    al[i] = vdot(Sa + (size_t)i * n, q, n) * ro[(size_t)a * m + i];
+                    for (k = 0; k < n; ++k) q[k] = q[k] - Ya[(size_t)i * n + k] * al[i];
+                }
+                for (k = 0; k < n; ++k) da[k] = q[k] * H[a];
+                for (i = 0; i < m; ++i) {
+                    if (i >= hl[a]) continue;
+                    double be = vdot(Ya + (size_t)i * n, da, n) * ro[(size_t)a * m + i];
+                    for (k = 0; k < n; ++k) da[k] = da[k] + Sa[(size_t)i * n + k] * (al[i] - be);
+                }
+            }
+            memcpy(gp + (size_t)a * n, ga, sizeof(double) * n);
+            lprev[a] = loss[a];
+            if (it == 1) {
+                double gs = 0.0;
+                for (k = 0; k < n; ++k) gs += fabs(ga[k]);
+                double tt = 1.0 / gs;
+                t[a] = (1.0 < tt ? 1.0 : tt) * lr;
+                if (!act[a]) t[a] = 0.0;
+            } else {
+                t[a] = lr;
+            }
+            if (max_step > 0.0) {
+                double pc = 0.0;
+                for (k = 0; k < n; ++k) pc = fmax(pc, fabs(da[k] * t[a]));
+                if (pc > max_step) t[a] *= max_step / pc;
+            }
+            double gtd = vdot(ga, da, n);
+            if (gtd > -tol_change) act[a] = 0;
+        }
+        any = 0;
+        for (a = 0; a < M; ++a) any |= act[a];
+        if (!any) break;
+        for (a = 0; a < M; ++a)
+            for (k = 0; k < n; ++k) x[(size_t)a * n + k] = x[(size_t)a * n + k] + t[a] * d[(size_t)a * n + k];
+        if (it != max_iter) f(ctx, x, M, n, loss, g);
+        for (a = 0; a < M; ++a) fe[a] += 1;
+        if (it > 1) {
+            for (a = 0; a < M; ++a) {
+                double *ga = g + (size_t)a * n, *gpa = gp + (size_t)a * n, *da = d + (size_t)a * n;
+                for (k = 0; k < n; ++k) { y[k] = ga[k] - gpa[k]; sv[k] = da[k] * t[a]; }
+                double ys = vdot(y, sv, n);
+                if (!(act[a] && ys > 1e-10)) continue;
+                H[a] = ys / vdot(y, y, n);
+                double *Ya = Y + (size_t)a * m * n, *Sa = S + (size_t)a * m * n;
+                int slot;
+                if (hl[a] < m) {
+                    slot = hl[a]++;
+                } else {
+                    memmove(Ya, Ya + n, sizeof(double) * (size_t)(m - 1) * n);
+                    memmove(Sa, Sa + n, sizeof(double) * (size_t)(m - 1) * n);
+                    memmove(ro + (size_t)a * m, ro + (size_t)a * m + 1, sizeof(double) * (m - 1));
+                    slot = m - 1;
+                }
+                memcpy(Ya + (size_t)slot * n, y, sizeof(double) * n);
+                memcpy(Sa + (size_t)slot * n, sv, sizeof(double) * n);
+                ro[(size_t)a * m + slot] = 1.0 / ys;
+            }
+        }
+        any = 0;
+        for (a = 0; a < M; ++a) {
+            double *ga = g + (size_t)a * n, *da = d + (size_t)a * n;
+            double gn = 0.0, pc = 0.0;
+            for (k = 0; k < n; ++k) { gn = fmax(gn, fabs(ga[k])); pc = fmax(pc, fabs(da[k] * t[a])); }
+            if (gn <= tol_grad) act[a] = 0;
+            if (fe[a] >= max_eval) act[a] = 0;
+            if (pc <= tol_change && act[a]) act[a] = 0;
+            double lc = fabs(loss[a] - lprev[a]);
+            if (lc < tol_change && act[a]) act[a] = 0;
+            if (tol_rel > 0 && it > m) {
+                double rel = lc / (fabs(lprev[a]) + 1e-12);
+                if (rel < tol_rel && act[a]) act[a] = 0;
+            }
+            if (stop_on_increase && loss[a] > l0[a] && act[a]) act[a] = 0;
+            any |= act[a];
+        }
+    }
+    memcpy(x0, x, sizeof(double) * Mn);
+    free(x); free(g); free(gp); free(d); free(q); free(y); free(sv); free(Y); free(S); free(ro);
+    free(al); free(loss); free(lprev); free(l0); free(H); free(t); free(hl); free(act); free(fe);
+    return it;
+}
+
+/* ---------------------------------------------------------------------------
+ * CPU relaxation baseline: cost = w_ov * overlap(area|sep) + w_bd * boundary, h fixed,
+ * state x = [xyt | h] per instance as core/pack_dynamics.py:76-83 flattens it.
+ * Each instance is its own L-BFGS run (M = 1), so the result does not depend on the
+ * batch; instances run in parallel over OpenMP threads.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const or_geom *g;
+    int n_trees, use_sep;
+    double w_ov, w_bd;
+} relax_ctx;
+
+static void relax_fn(void *vc, const double *x, int M, int n, double *loss, double *grad) {
+    relax_ctx *c = (relax_ctx *)vc;
+    int a, k;
+    int N = c->n_trees;
+    double *gb = (double *)malloc(sizeof(double) * 3 * N), *gh = (double *)malloc(sizeof(double) * 3);
+    for (a = 0; a < M; ++a) {
+        const double *xa = x + (size_t)a * n;
+        double *ga = grad + (size_t)a * n;
+        double co, cb;
+        or_overlap(c->g, xa, 1, N, c->use_sep, NULL, 0, 0, &co, ga);
+        or_boundary(c->g, xa, xa + 3 * N, 1, N, &cb, gb, gh);
+        for (k = 0; k < 3 * N; ++k) ga[k] = c->w_ov * ga[k] + c->w_bd * gb[k];
+        ga[3 * N] = ga[3 * N + 1] = ga[3 * N + 2] = 0.0;   /* fixed h */
+        loss[a] = c->w_ov * co + c->w_bd * cb;
+    }
+    free(gb); free(gh);
+}
+
+void or_relax(const or_geom *g, double *xyt, const double *h, int n_ens, int n_trees, int use_sep,
+              double w_ov, double w_bd, int max_iter, int m, double max_step, double lr,
+              int n_threads) {
+    int e;
+    int n = 3 * n_trees + 3;
+#ifdef _OPENMP
+    #pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads > 0 ? n_threads : 1)
+#endif
+    for (e = 0; e < n_ens; ++e) {
+        relax_ctx c;
+        c.g = g; c.n_trees = n_trees; c.use_sep = use_sep; c.w_ov = w_ov; c.w_bd = w_bd;
+        double *x = (double *)malloc(sizeof(double) * n);
+        memcpy(x, xyt + (size_t)e * 3 * n_trees, sizeof(double) * 3 * n_trees);
+        memcpy(x + 3 * n_trees, h + 3 * e, sizeof(double) * 3);
+        or_lbfgs(relax_fn, &c, x, 1, n, lr, max_iter, 0, 0.0, 0.0, 0.0, 0, m, max_step);
+        memcpy(xyt + (size_t)e * 3 * n_trees, x, sizeof(double) * 3 * n_trees);
+        free(x);
+    }
+    (void)n_threads;
+}
