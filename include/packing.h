@@ -1,0 +1,163 @@
+/*
+ * packing.h -- C-ABI of libpacking.so, the MI355X (gfx950) relaxation core.
+ *
+ * Conventions (mirroring the reference launchers):
+ *   - every function returns PK_OK (0) or a negative status; pk_last_error() gives the
+ *     message for the calling thread;
+ *   - array arguments are device pointers (C-contiguous), sizes are plain ints, and
+ *     `stream` is a hipStream_t (NULL = default stream); launches are asynchronous and
+ *     nothing synchronises on return, as with the reference's CuPy launches;
+ *   - the caller allocates every output; the callee writes all of it (zeroing what it
+ *     accumulates), as core/pack_cuda.py:1139-1142 does;
+ *   - n_ens == 0 returns immediately (core/pack_cuda.py:1075-1076).
+ * Poses are (x, y, theta) triples; `xyt` of a uniform batch is (n_ens, n_trees, 3).
+ * A ragged batch passes inst_off (n_ens + 1 int32 tree offsets, device) and
+ * n_trees = the largest instance size.
+ */
+#ifndef PACKING_H
+#define PACKING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_ABI_VERSION 1
+
+#define PK_OK 0
+#define PK_F32 0
+#define PK_F64 1
+#define PK_MODE_AREA 0
+#define PK_MODE_SEP 1
+#define PK_MODE_LUT 2
+#define PK_MAX_HISTORY 16
+#define PK_LBFGS_SLOT_BYTES 184
+
+typedef struct pk_geometry_opaque* pk_geometry_t;
+typedef struct pk_lut_opaque* pk_lut_t;
+
+const char* pk_last_error(void);
+int pk_version(void);
+
+/* Device properties used by the host-side tile choice. */
+int pk_device_info(int* max_lds_per_block, int* n_cu, int* clock_khz);
+
+/*
+ * Polygon as convex pieces (local frame, CCW, padded by repeating the last vertex) plus
+ * its outline vertex list and bounding radius.  Replaces the constant-memory upload of
+ * core/pack_cuda.py:1005-1030 (pieces from _build_convex_piece_arrays, :787-849; tree
+ * vertices from core/kaggle_support.py:351-354) with a handle, so several polygons can
+ * coexist.  1 <= n_pieces <= 16, 3 <= vertices per piece <= max_verts <= 4,
+ * n_outline <= 64.
+ */
+int pk_geometry_create(const double* piece_xy, const int32_t* piece_nverts, int n_pieces,
+                       int max_verts, const double* outline_xy, int n_outline, double radius,
+                       pk_geometry_t* out);
+int pk_geometry_destroy(pk_geometry_t g);
+
+/*
+ * Exact pairwise cost: mode PK_MODE_AREA = summed overlap area of all tree pairs,
+ * PK_MODE_SEP = summed squared SAT penetration depth of all piece pairs.
+ * Replaces pack_cuda.overlap_multi_ensemble (core/pack_cuda.py:1040-1183) and its
+ * kernel multi_overlap_list_total (:482-546).  out_grads (n_ens, n_trees, 3) may be NULL
+ * (cost only).  crystal (n_ens, 4) = lattice axes [ax, ay, bx, by] or NULL; with it the
+ * sum runs over 0 <= |cell| <= n_periodic images, and only_self keeps only a tree's own
+ * images (core/pack_cuda.py:332-364).  block = 64, 128 or 256 threads per instance
+ * (0 = 256).  stats (2 uint64, device, accumulated) = [tree pairs within the cutoff,
+ * piece pairs clipped], or NULL.
+ */
+int pk_overlap(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst_off, int n_ens,
+               int n_trees, int mode, const void* crystal, int n_periodic, int only_self,
+               void* out_cost, void* out_grads, int block, unsigned long long* stats,
+               void* stream);
+
+/*
+ * Boundary term: per tree, the max over outline vertices of the squared distance
+ * outside the square h = [size, x_centre, y_centre]; gradients w.r.t. poses and h.
+ * Replaces pack_cuda.boundary_distance_multi_ensemble (core/pack_cuda.py:1186-1295).
+ * h is (n_ens, 3); out_grads and out_grad_h may be NULL.
+ */
+int pk_boundary(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst_off, int n_ens,
+                int n_trees, const void* h, void* out_cost, void* out_grads, void* out_grad_h,
+                void* stream);
+
+/*
+ * Pose-space lookup table (float32, vals[ix][iy][it] with theta fastest, device memory
+ * owned by the caller).  Replaces pack_cuda_lut.LookupTable's device part and the
+ * constant-memory cache keyed by id(lut) (core/pack_cuda_lut.py:30-89,653-695,811-815)
+ * with an explicit handle.  mirror = 1 reproduces the reference's half table (x >= 0
+ * stored, x < 0 looked up as (-x, -theta); core/pack_cuda_lut.py:128-131,314-320);
+ * mirror = 0 is a full-range table for polygons without mirror symmetry.
+ * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).
+ */
+int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, double x_max,
+                  double y_min, double y_max, double t_min, double t_max, int quad, int mirror,
+                  pk_lut_t* out);
+int pk_lut_destroy(pk_lut_t lut);
+
+/*
+ * Fills vals (nx*ny*nt, float32, device) with the exact overlap area of the polygon at
+ * the origin (theta = 0) and the polygon at (X[ix], Y[iy], T[it]).  Replaces
+ * LookupTable.build_from_function driven by the exact kernel
+ * (core/pack_cuda_lut.py:91-169, core/pack_cost.py:322-377).
+ */
+int pk_lut_build(pk_geometry_t g, const float* X, int nx, const float* Y, int ny, const float* T,
+                 int nt, float* out_vals, void* stream);
+
+/*
+ * Table-based cost: sum over tree pairs i<j within (2R) of the trilinear table value of
+ * j's pose relative to i, gradient to both trees.  Replaces
+ * pack_cuda_lut.overlap_multi_ensemble (core/pack_cuda_lut.py:783-862) and its kernel
+ * multi_overlap_lut_total (:596-640).  float32 only; out_grads may be NULL.
+ */
+int pk_lut_overlap(pk_geometry_t g, pk_lut_t lut, const float* xyt, const int32_t* inst_off,
+                   int n_ens, int n_trees, float* out_cost, float* out_grads, int block,
+                   unsigned long long* stats, void* stream);
+
+/*
+ * Fused relaxation engine (no reference counterpart as one kernel: it fuses the cost of
+ * core/pack_dynamics.py:93-123 with core/lbfgs_torch_parallel.py:411-729, no line
+ * search).  Each slot holds one instance: state x = [xyt (N,3) | h (3)] at x + xoff[s],
+ * its gradient at g + xoff[s], L-BFGS history at s_hist/y_hist + history*xoff[s], and a
+ * PK_LBFGS_SLOT_BYTES record in `state` (all zero = fresh slot: the first call
+ * evaluates the cost and starts the run).  One call advances every listed slot by up to
+ * `iters` iterations; a slot stops for good when it converges or reaches max_iter, with
+ * the reference's tests applied to it alone (batch-independent).  Workgroup b runs slot
+ * order[b] (order NULL: b).  status_out[s] = 1 once slot s is finished.
+ */
+typedef struct {
+    int n_slots;
+    const int32_t* xoff;       /* [n_slots] state offset in floats                   */
+    const int32_t* ntree;      /* [n_slots] trees per slot (0: empty slot, skipped)    */
+    float* x;                  /* state                                               */
+    float* g;                  /* gradient                                            */
+    float* s_hist;             /* history * state floats per slot                     */
+    float* y_hist;
+    void* state;               /* [n_slots] PK_LBFGS_SLOT_BYTES records               */
+    const int32_t* order;      /* [n_launch] slot per workgroup, or NULL              */
+    int n_launch;
+    int n_tree_max;            /* largest ntree of the launched slots (LDS sizing)    */
+    int mode;                  /* PK_MODE_AREA, PK_MODE_SEP or PK_MODE_LUT            */
+    float w_overlap, w_boundary, w_area;
+    int fixed_h;
+    pk_lut_t lut;              /* PK_MODE_LUT only                                    */
+    int history, max_iter, max_eval;   /* max_eval <= 0: max_iter * 5 / 4            */
+    float lr, max_step, tol_grad, tol_change, tol_rel;   /* max_step <= 0: no clamp  */
+    int stop_on_increase;
+    int iters;                 /* iterations this call                                */
+    int block;                 /* 64, 128, 256 (0 = 256)                              */
+    int32_t* status_out;       /* [n_slots] or NULL                                   */
+    float* loss_out;           /* [n_slots] or NULL                                   */
+    unsigned long long* stats; /* [2] tree pairs tested-in, pairs clipped/looked up   */
+} pk_relax_params;
+
+int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
+int pk_relax_slot_bytes(void);
+int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PACKING_H */

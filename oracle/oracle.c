@@ -870,6 +870,7 @@ typedef struct {
     const or_geom *g;
     int n_trees, use_sep;
     double w_ov, w_bd;
+    const or_lut *lut;   /* non-NULL: table-based overlap term */
 } relax_ctx;
 
 static void relax_fn(void *vc, const double *x, int M, int n, double *loss, double *grad) {
@@ -881,7 +882,10 @@ static void relax_fn(void *vc, const double *x, int M, int n, double *loss, doub
         const double *xa = x + (size_t)a * n;
         double *ga = grad + (size_t)a * n;
         double co, cb;
-        or_overlap(c->g, xa, 1, N, c->use_sep, NULL, 0, 0, &co, ga);
+        if (c->lut)
+            or_lut_overlap(c->lut, sqrt(c->g->cutoff_sq) * 0.5, xa, 1, N, &co, ga);
+        else
+            or_overlap(c->g, xa, 1, N, c->use_sep, NULL, 0, 0, &co, ga);
         or_boundary(c->g, xa, xa + 3 * N, 1, N, &cb, gb, gh);
         for (k = 0; k < 3 * N; ++k) ga[k] = c->w_ov * ga[k] + c->w_bd * gb[k];
         ga[3 * N] = ga[3 * N + 1] = ga[3 * N + 2] = 0.0;   /* fixed h */
@@ -890,9 +894,9 @@ static void relax_fn(void *vc, const double *x, int M, int n, double *loss, doub
     free(gb); free(gh);
 }
 
-void or_relax(const or_geom *g, double *xyt, const double *h, int n_ens, int n_trees, int use_sep,
-              double w_ov, double w_bd, int max_iter, int m, double max_step, double lr,
-              int n_threads) {
+void or_relax(const or_geom *g, const or_lut *lut, double *xyt, const double *h, int n_ens,
+              int n_trees, int use_sep, double w_ov, double w_bd, int max_iter, int m,
+              double max_step, double lr, int n_threads) {
     int e;
     int n = 3 * n_trees + 3;
 #ifdef _OPENMP
@@ -900,7 +904,7 @@ void or_relax(const or_geom *g, double *xyt, const double *h, int n_ens, int n_t
 #endif
     for (e = 0; e < n_ens; ++e) {
         relax_ctx c;
-        c.g = g; c.n_trees = n_trees; c.use_sep = use_sep; c.w_ov = w_ov; c.w_bd = w_bd;
+        c.g = g; c.n_trees = n_trees; c.use_sep = use_sep; c.w_ov = w_ov; c.w_bd = w_bd; c.lut = lut;
         double *x = (double *)malloc(sizeof(double) * n);
         memcpy(x, xyt + (size_t)e * 3 * n_trees, sizeof(double) * 3 * n_trees);
         memcpy(x + 3 * n_trees, h + 3 * e, sizeof(double) * 3);

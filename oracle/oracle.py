@@ -54,8 +54,9 @@ def lib():
         L.or_feasible.restype = None
         L.or_feasible.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, ip]
         L.or_relax.restype = None
-        L.or_relax.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double,
-                               C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.or_relax.argtypes = [C.c_void_p, C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_int,
+                               C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                               C.c_int]
         FN = C.CFUNCTYPE(None, C.c_void_p, dp, C.c_int, C.c_int, dp, dp)
         L._FN = FN
         L.or_lbfgs.restype = C.c_int
@@ -203,12 +204,14 @@ def feasible_pairs(og: OracleGeometry, xyt):
 
 
 def relax(og: OracleGeometry, xyt, h, use_sep=False, w_ov=1.0, w_bd=1.0, max_iter=200, m=8,
-          max_step=0.5, lr=1.0, n_threads=1):
-    """Per-instance L-BFGS relaxation in FP64 (returns relaxed xyt)."""
+          max_step=0.5, lr=1.0, n_threads=1, lut: "OracleLut | None" = None):
+    """Per-instance L-BFGS relaxation in FP64 (returns relaxed xyt); lut selects the
+    table-based overlap term."""
     x = np.ascontiguousarray(xyt, dtype=np.float64).copy()
     h = np.ascontiguousarray(h, dtype=np.float64)
     E, N = x.shape[:2]
-    lib().or_relax(og.ptr, _dp(x), _dp(h), E, N, int(use_sep), w_ov, w_bd, max_iter, m,
+    lib().or_relax(og.ptr, None if lut is None else C.byref(lut.s), _dp(x), _dp(h), E, N,
+                   int(use_sep), w_ov, w_bd, max_iter, m,
                    max_step if max_step is not None else 0.0, lr, n_threads)
     return x
 

@@ -1,0 +1,130 @@
+"""ctypes binding of libpacking.so (C-ABI declared in include/packing.h).
+
+The library must have been built (packing_amd/_build.py); there is no fallback: every
+entry point raises if the native library cannot be loaded.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpacking.so")
+
+PK_F32, PK_F64 = 0, 1
+PK_MODE_AREA, PK_MODE_SEP, PK_MODE_LUT = 0, 1, 2
+PK_MAX_HISTORY = 16
+PK_LBFGS_SLOT_BYTES = 184
+
+_lib = None
+_lock = threading.Lock()
+
+vp = C.c_void_p
+i32 = C.c_int
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pk_last_error": (C.c_char_p, []),
+    "pk_version": (i32, []),
+    "pk_device_info": (i32, [C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "pk_geometry_create": (i32, [C.POINTER(C.c_double), C.POINTER(C.c_int32), i32, i32,
+                                 C.POINTER(C.c_double), i32, C.c_double, C.POINTER(vp)]),
+    "pk_geometry_destroy": (i32, [vp]),
+    "pk_overlap": (i32, [vp, i32, vp, vp, i32, i32, i32, vp, i32, i32, vp, vp, i32, vp, vp]),
+    "pk_boundary": (i32, [vp, i32, vp, vp, i32, i32, vp, vp, vp, vp, vp]),
+    "pk_lut_create": (i32, [vp, i32, i32, i32, C.c_double, C.c_double, C.c_double, C.c_double,
+                            C.c_double, C.c_double, i32, i32, C.POINTER(vp)]),
+    "pk_lut_destroy": (i32, [vp]),
+    "pk_lut_build": (i32, [vp, vp, i32, vp, i32, vp, i32, vp, vp]),
+    "pk_lut_overlap": (i32, [vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp]),
+    "pk_relax_step": (i32, [vp, vp, vp]),
+    "pk_relax_slot_bytes": (i32, []),
+    "pk_relax_lds_bytes": (i32, [vp, i32, i32]),
+}
+
+
+class RelaxParams(C.Structure):
+    """ctypes mirror of pk_relax_params (include/packing.h)."""
+    _fields_ = [
+        ("n_slots", i32), ("xoff", vp), ("ntree", vp), ("x", vp), ("g", vp), ("s_hist", vp),
+        ("y_hist", vp), ("state", vp), ("order", vp), ("n_launch", i32), ("n_tree_max", i32),
+        ("mode", i32), ("w_overlap", C.c_float), ("w_boundary", C.c_float), ("w_area", C.c_float),
+        ("fixed_h", i32), ("lut", vp), ("history", i32), ("max_iter", i32), ("max_eval", i32),
+        ("lr", C.c_float), ("max_step", C.c_float), ("tol_grad", C.c_float),
+        ("tol_change", C.c_float), ("tol_rel", C.c_float), ("stop_on_increase", i32),
+        ("iters", i32), ("block", i32), ("status_out", vp), ("loss_out", vp), ("stats", vp),
+    ]
+
+
+class PackingError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise PackingError(
+                    f"{LIB_PATH} is missing: build it with `python -m packing_amd._build` "
+                    "(or __graft_entry__.build())")
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGS.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc == 0:
+        return
+    msg = lib().pk_last_error().decode(errors="replace")
+    if rc == -1:
+        raise ValueError(f"{what}: {msg}")
+    if rc == -3:
+        raise MemoryError(f"{what}: {msg}")
+    if rc == -4:
+        raise ValueError(f"{what}: {msg}")
+    raise PackingError(f"{what}: {msg} (status {rc})")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(stream=None) -> int | None:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    h = getattr(stream, "cuda_stream", None)
+    if h is None:
+        h = int(stream)
+    return h or None
+
+
+def dtype_code(dtype) -> int:
+    import torch
+    if dtype == torch.float32:
+        return PK_F32
+    if dtype == torch.float64:
+        return PK_F64
+    raise ValueError(f"unsupported dtype {dtype}; float32 or float64")
+
+
+def device_info():
+    a, b, c = i32(), i32(), i32()
+    check(lib().pk_device_info(C.byref(a), C.byref(b), C.byref(c)), "pk_device_info")
+    return {"max_lds_per_block": a.value, "n_cu": b.value, "clock_khz": c.value}

@@ -1,0 +1,661 @@
+// Per-instance evaluation of the exact overlap / separation cost and its gradient.
+//
+// One workgroup evaluates one packing instance.  Stages (all in LDS):
+//   1. poses -> sincos -> piece vertices rotated about their tree centre;
+//   2. tree-pair candidates i<j (round-robin order, uniform work per thread) tested
+//      against the bounding-circle cutoff (2R)^2 and compacted with wave ballots;
+//   3. piece pairs of surviving tree pairs tested against piece bounding circles and
+//      compacted the same way;
+//   4. one lane per piece pair: exact clipped area (or SAT depth) and the gradient for
+//      BOTH trees, accumulated as int64 fixed point (order independent).
+//
+// Semantics follow core/pack_cuda.py:142-546: self pairs skipped, cutoff
+// |dp|^2 <= (2R)^2, total = sum over unordered pairs (the reference halves its ordered
+// sum), separation cost = sum of sep^2 with sep from the SAT of
+// core/pack_cuda_primitives.py:544-828, periodic images as core/pack_cuda.py:332-364
+// (self images give cost only).  The area itself is computed differently from the
+// reference (Sutherland-Hodgman with reverse mode there; Green's theorem over clipped
+// edges with shape-derivative gradients here): identical value, no dynamic arrays.
+#pragma once
+#include "pk_common.h"
+
+namespace pk {
+
+enum { MODE_AREA = 0, MODE_SEP = 1, MODE_LUT = 2 };
+
+// Pose-space lookup table in device memory: vals[ix][iy][it], theta fastest.
+struct LutDev {
+    const float* vals;
+    int nx, ny, nt;
+    float x_lo, x_min, x_max, y_min, y_max, t_min, t_max;   // x_lo = range check low end
+    float sx, sy, st;                                       // (n - 1) / (max - min)
+    int quad;      // per-pair max(0, v)^2
+    int mirror;    // table covers x >= 0; x < 0 looked up as (-x, -theta)
+};
+
+// Trilinear value and analytic gradient w.r.t. (x, y, theta) of the table
+// (core/pack_cuda_lut.py:298-415): outside the grid range 0; clamped upper corner.
+__device__ inline float lut_lookup(const LutDev& T_, float x, float y, float th, float (&g)[3]) {
+    g[0] = g[1] = g[2] = 0.f;
+    if (x < T_.x_lo || x > T_.x_max || y < T_.y_min || y > T_.y_max || th < T_.t_min || th > T_.t_max)
+        return 0.f;
+    const bool flip = T_.mirror && x < 0.f;
+    const float xt = flip ? -x : x, tt = flip ? -th : th;
+    const float gx = (xt - T_.x_min) * T_.sx, gy = (y - T_.y_min) * T_.sy, gt = (tt - T_.t_min) * T_.st;
+    const int ix0 = (int)floorf(gx), iy0 = (int)floorf(gy), it0 = (int)floorf(gt);
+    const int ix1 = min(ix0 + 1, T_.nx - 1), iy1 = min(iy0 + 1, T_.ny - 1), it1 = min(it0 + 1, T_.nt - 1);
+    const float fx = gx - ix0, fy = gy - iy0, ft = gt - it0;
+    const float* __restrict__ v = T_.vals;
+    const size_t r00 = ((size_t)ix0 * T_.ny + iy0) * T_.nt, r01 = ((size_t)ix0 * T_.ny + iy1) * T_.nt;
+    const size_t r10 = ((size_t)ix1 * T_.ny + iy0) * T_.nt, r11 = ((size_t)ix1 * T_.ny + iy1) * T_.nt;
+    const float v000 = v[r00 + it0], v001 = v[r00 + it1], v010 = v[r01 + it0], v011 = v[r01 + it1];
+    const float v100 = v[r10 + it0], v101 = v[r10 + it1], v110 = v[r11 + it0], v111 = v[r11 + it1];
+    const float v00 = v000 * (1.f - ft) + v001 * ft, v01 = v010 * (1.f - ft) + v011 * ft;
+    const float v10 = v100 * (1.f - ft) + v101 * ft, v11 = v110 * (1.f - ft) + v111 * ft;
+    const float v0 = v00 * (1.f - fy) + v01 * fy, v1 = v10 * (1.f - fy) + v11 * fy;
+    const float val = v0 * (1.f - fx) + v1 * fx;
+    const float d0 = (v001 - v000) * (1.f - fy) + (v011 - v010) * fy;
+    const float d1 = (v101 - v100) * (1.f - fy) + (v111 - v110) * fy;
+    g[0] = (v1 - v0) * T_.sx;
+    g[1] = ((1.f - fx) * (v01 - v00) + fx * (v11 - v10)) * T_.sy;
+    g[2] = (d0 * (1.f - fx) + d1 * fx) * T_.st;
+    if (flip) { g[0] = -g[0]; g[2] = -g[2]; }
+    return val;
+}
+
+// wrap to [-pi, pi] (core/pack_cuda_lut.py:289-293)
+__device__ inline float wrap_pi(float t) {
+    const float PI = 3.14159265358979323846f;
+    while (t > PI) t -= 2.f * PI;
+    while (t < -PI) t += 2.f * PI;
+    return t;
+}
+
+// Copy the polygon tables into LDS (all threads of the block).
+template <typename T>
+__device__ inline void load_geom(GeomT<T>& G, const GeomT<T>* __restrict__ src) {
+    const int* s = reinterpret_cast<const int*>(src);
+    int* d = reinterpret_cast<int*>(&G);
+    for (int q = threadIdx.x; q < (int)(sizeof(GeomT<T>) / sizeof(int)); q += blockDim.x) d[q] = s[q];
+}
+
+constexpr int TP_CAP = 1024;   // tree-pair list capacity (entries)
+constexpr int PP_CAP = 2048;   // piece-pair list capacity (entries)
+
+// LDS view for one instance.
+template <typename T>
+struct EvalLds {
+    using A = typename Acc<T>::type;
+    A* g;          // [N*3] gradient accumulators
+    A* cost;       // [1]   cost accumulator
+    T* x;          // [N]   poses
+    T* y;
+    T* t;
+    T* c;          // [N]   cos, sin of theta
+    T* s;
+    T* vx;         // [N*P*MAXV] piece vertices rotated about the tree centre
+    T* vy;
+    uint32_t* tp;  // [TP_CAP] tree-pair entries: i | j << 12 | cell << 24
+    uint32_t* pp;  // [PP_CAP] piece-pair entries: tp index | pi << 16 | pj << 20
+    int* wcnt;     // [16] per-wave ballot counts
+    int N, P;
+};
+
+__host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+template <typename T>
+__host__ __device__ inline size_t eval_lds_bytes(int N, int P) {
+    size_t b = 0;
+    b += sizeof(typename Acc<T>::type) * (3 * (size_t)N + 1);
+    b = align_up(b, 16);
+    b += sizeof(T) * 5 * (size_t)N;
+    b = align_up(b, 16);
+    b += sizeof(T) * 2 * (size_t)N * P * MAXV;
+    b = align_up(b, 16);
+    b += sizeof(uint32_t) * (TP_CAP + PP_CAP);
+    b += sizeof(int) * 16;
+    return align_up(b, 16);
+}
+
+template <typename T>
+__device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P) {
+    EvalLds<T> L;
+    size_t o = 0;
+    L.g = reinterpret_cast<typename Acc<T>::type*>(base + o);
+    L.cost = L.g + 3 * N;
+    o += sizeof(typename Acc<T>::type) * (3 * (size_t)N + 1);
+    o = align_up(o, 16);
+    T* f = reinterpret_cast<T*>(base + o);
+    L.x = f; L.y = f + N; L.t = f + 2 * N; L.c = f + 3 * N; L.s = f + 4 * N;
+    o += sizeof(T) * 5 * (size_t)N;
+    o = align_up(o, 16);
+    L.vx = reinterpret_cast<T*>(base + o);
+    L.vy = L.vx + (size_t)N * P * MAXV;
+    o += sizeof(T) * 2 * (size_t)N * P * MAXV;
+    o = align_up(o, 16);
+    L.tp = reinterpret_cast<uint32_t*>(base + o);
+    L.pp = L.tp + TP_CAP;
+    L.wcnt = reinterpret_cast<int*>(L.pp + PP_CAP);
+    L.N = N; L.P = P;
+    return L;
+}
+
+// ---------------------------------------------------------------------------------
+// Geometry of one piece pair
+// ---------------------------------------------------------------------------------
+
+// Clip every edge of polygon p against the half-planes of convex polygon q (both CCW,
+// padded to MAXV by repeated vertices).  Edge k keeps the parameter range
+// [t0[k], t1[k]] of p_k + t (p_{k+1} - p_k); t1 <= t0 means nothing is inside.
+// Exactly collinear edge pairs are counted once: on p's side only when `keep_same`
+// is set and the two edges run the same way.
+//
+// The side-of-line values f = cross(q_{m+1} - q_m, p - q_m) are evaluated in double
+// with separately rounded products (no fused multiply-add): for float vertices the
+// differences are exact and a truly collinear point gives exactly 0, so both polygons
+// are classified consistently (A's edges against B's lines and B's against A's).
+// Green's sum then describes one closed boundary even at exact contacts.
+template <typename T>
+__host__ __device__ inline void clip_edges(const T (&px)[MAXV], const T (&py)[MAXV], const T (&qx)[MAXV],
+                                  const T (&qy)[MAXV], bool keep_same, T (&t0)[MAXV],
+                                  T (&t1)[MAXV]) {
+#pragma clang fp contract(off)
+    T qex[MAXV], qey[MAXV];
+    double dqx[MAXV], dqy[MAXV], dex[MAXV], dey[MAXV];
+    bool qdeg[MAXV];
+#pragma unroll
+    for (int m = 0; m < MAXV; ++m) {
+        const int m1 = (m + 1) % MAXV;
+        qex[m] = qx[m1] - qx[m];
+        qey[m] = qy[m1] - qy[m];
+        dqx[m] = (double)qx[m];
+        dqy[m] = (double)qy[m];
+        dex[m] = (double)qx[m1] - dqx[m];
+        dey[m] = (double)qy[m1] - dqy[m];
+        qdeg[m] = (dex[m] == 0.0) & (dey[m] == 0.0);
+    }
+    T f[MAXV][MAXV];
+#pragma unroll
+    for (int v = 0; v < MAXV; ++v) {
+        const double pxd = (double)px[v], pyd = (double)py[v];
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const double a = dex[m] * (pyd - dqy[m]);
+            const double b = dey[m] * (pxd - dqx[m]);
+            const double val = a - b;
+            f[v][m] = qdeg[m] ? T(1) : (T)val;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        const int k1 = (k + 1) % MAXV;
+        const T ex = px[k1] - px[k], ey = py[k1] - py[k];
+        T lo = T(0);
+        T hi = ((ex == T(0)) & (ey == T(0))) ? T(0) : T(1);
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const T f0 = f[k][m], f1 = f[k1][m];
+            const bool coll = (f0 == T(0)) & (f1 == T(0));
+            const bool same = keep_same & (ex * qex[m] + ey * qey[m] > T(0));
+            const bool out = ((f0 < T(0)) & (f1 < T(0))) | (coll & !same);
+            const T tt = f0 * fast_rcp(f0 - f1);
+            const bool leave = (f0 >= T(0)) & (f1 < T(0));
+            const bool enter = (f0 < T(0)) & (f1 >= T(0));
+            hi = leave ? fmin(hi, tt) : hi;
+            lo = enter ? fmax(lo, tt) : lo;
+            hi = out ? T(-1) : hi;
+        }
+        t0[k] = lo;
+        t1[k] = hi;
+    }
+}
+
+// Sums over the clipped edges of polygon p: twice the signed area term
+// sum dt * cross(p_k, e_k), the chain displacement, the segment midpoints, and the
+// shape derivative of the area w.r.t. a rigid motion of p about centre (cx, cy):
+//   d/dx = sum dt e_y,  d/dy = -sum dt e_x,  d/dtheta = -sum dt (mid - c) . e
+template <typename T, bool GRAD>
+__host__ __device__ inline void edge_sums(const T (&px)[MAXV], const T (&py)[MAXV], const T (&t0)[MAXV],
+                                 const T (&t1)[MAXV], T cx, T cy, T& a2, T& Dx, T& Dy, T& mx,
+                                 T& my, int& cnt, T (&g)[3]) {
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+        const int k1 = (k + 1) % MAXV;
+        const T ex = px[k1] - px[k], ey = py[k1] - py[k];
+        const bool valid = t1[k] > t0[k];
+        const T dt = valid ? t1[k] - t0[k] : T(0);
+        const T tm = T(0.5) * (t0[k] + t1[k]);
+        const T midx = px[k] + tm * ex, midy = py[k] + tm * ey;
+        a2 += dt * (px[k] * ey - py[k] * ex);
+        Dx += dt * ex;
+        Dy += dt * ey;
+        mx += valid ? midx : T(0);
+        my += valid ? midy : T(0);
+        cnt += valid ? 1 : 0;
+        if (GRAD) {
+            g[0] += dt * ey;
+            g[1] -= dt * ex;
+            g[2] -= dt * ((midx - cx) * ex + (midy - cy) * ey);
+        }
+    }
+}
+
+// Area of A ∩ B (A about the origin = centre of tree i, B about (djx, djy) = centre of
+// tree j) and its gradient w.r.t. both poses.  The Green's-theorem sum is taken about
+// the mean of the clipped segment midpoints, which keeps near-collinear contacts (where
+// rounding can count an edge on one side only) accurate to rounding.
+template <typename T, bool GRAD>
+__host__ __device__ inline T area_pair(const T (&ax)[MAXV], const T (&ay)[MAXV], const T (&bx)[MAXV],
+                              const T (&by)[MAXV], T djx, T djy, T (&gi)[3], T (&gj)[3]) {
+    T t0[MAXV], t1[MAXV];
+    T a2 = 0, Dx = 0, Dy = 0, mx = 0, my = 0;
+    int cnt = 0;
+    clip_edges<T>(ax, ay, bx, by, true, t0, t1);
+    edge_sums<T, GRAD>(ax, ay, t0, t1, T(0), T(0), a2, Dx, Dy, mx, my, cnt, gi);
+    clip_edges<T>(bx, by, ax, ay, false, t0, t1);
+    edge_sums<T, GRAD>(bx, by, t0, t1, djx, djy, a2, Dx, Dy, mx, my, cnt, gj);
+    if (cnt > 0) {
+        const T inv = T(1) / T(cnt);
+        const T ox = mx * inv, oy = my * inv;
+        a2 -= ox * Dy - oy * Dx;
+    }
+    return T(0.5) * a2;
+}
+
+// SAT penetration depth of two convex pieces and its gradient w.r.t. both poses
+// (value and critical-vertex choice as core/pack_cuda_primitives.py:544-828, axes of
+// A first; gradient for B derived the same way with the roles of the axes swapped).
+template <typename T, bool GRAD>
+__host__ __device__ inline T sat_pair(const T (&ax)[MAXV], const T (&ay)[MAXV], const T (&bx)[MAXV],
+                             const T (&by)[MAXV], T djx, T djy, T (&gi)[3], T (&gj)[3]) {
+    T best = T(1e30);
+    bool separated = false;
+    bool case1 = true, from_a = true;
+    T bnx = 0, bny = 0, vx = 0, vy = 0, wx = 0, wy = 0;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k) {
+            const int k1 = (k + 1) % MAXV;
+            const T ex = pass == 0 ? ax[k1] - ax[k] : bx[k1] - bx[k];
+            const T ey = pass == 0 ? ay[k1] - ay[k] : by[k1] - by[k];
+            T nx = -ey, ny = ex;
+            const T len = sqrt(nx * nx + ny * ny);
+            const bool skip = len < T(1e-12);
+            const T il = skip ? T(0) : T(1) / len;
+            nx *= il;
+            ny *= il;
+            T mn1 = ax[0] * nx + ay[0] * ny, mx1 = mn1;
+            int imn1 = 0, imx1 = 0;
+#pragma unroll
+            for (int v = 1; v < MAXV; ++v) {
+                const T p = ax[v] * nx + ay[v] * ny;
+                if (p < mn1) { mn1 = p; imn1 = v; }
+                if (p > mx1) { mx1 = p; imx1 = v; }
+            }
+            T mn2 = bx[0] * nx + by[0] * ny, mx2 = mn2;
+            int imn2 = 0, imx2 = 0;
+#pragma unroll
+            for (int v = 1; v < MAXV; ++v) {
+                const T p = bx[v] * nx + by[v] * ny;
+                if (p < mn2) { mn2 = p; imn2 = v; }
+                if (p > mx2) { mx2 = p; imx2 = v; }
+            }
+            if (!skip) {
+                separated |= (mx1 < mn2) | (mx2 < mn1);
+                const T pen1 = mx1 - mn2, pen2 = mx2 - mn1;
+                const T pen = pen1 < pen2 ? pen1 : pen2;
+                if (pen < best) {
+                    best = pen;
+                    bnx = nx; bny = ny;
+                    from_a = pass == 0;
+                    case1 = pen1 <= pen2;
+                    const int iv = case1 ? imx1 : imn1;
+                    const int iw = case1 ? imn2 : imx2;
+                    // select by unrolled compare (no dynamic register indexing)
+#pragma unroll
+                    for (int v = 0; v < MAXV; ++v) {
+                        if (v == iv) { vx = ax[v]; vy = ay[v]; }
+                        if (v == iw) { wx = bx[v]; wy = by[v]; }
+                    }
+                }
+            }
+        }
+    }
+    if (separated || !(best > T(1e-12))) return T(0);
+    if (GRAD) {
+        const T sg = case1 ? T(1) : T(-1);
+        const T dvx = vx - wx, dvy = vy - wy;
+        const T axis = sg * (dvx * (-bny) + dvy * bnx);       // s (v - w) . J n
+        gi[0] += sg * bnx;
+        gi[1] += sg * bny;
+        gi[2] += sg * ((-vy) * bnx + vx * bny) + (from_a ? axis : T(0));
+        const T rwx = wx - djx, rwy = wy - djy;
+        gj[0] -= sg * bnx;
+        gj[1] -= sg * bny;
+        gj[2] += -sg * ((-rwy) * bnx + rwx * bny) + (from_a ? T(0) : axis);
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------------
+// Workgroup-level pipeline
+// ---------------------------------------------------------------------------------
+
+// Ordered block-wide append.  Every thread of the block must call it (uniform).
+// Returns this thread's slot offset (valid only if pred) and the block total.
+template <int BLOCK>
+__device__ inline int block_append(bool pred, int* wcnt, int& total) {
+    constexpr int NW = BLOCK / WAVE;
+    const uint64_t m = __ballot(pred);
+    const int wave = threadIdx.x / WAVE;
+    const int rank = __popcll(m & lanemask_lt());
+    if (__lane_id() == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        const int c = wcnt[w];
+        off += w < wave ? c : 0;
+        tot += c;
+    }
+    total = tot;
+    return off + rank;
+}
+
+struct Crystal {
+    float ax, ay, bx, by;
+    int n_periodic;
+    int only_self;
+};
+
+template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK>
+struct Evaluator {
+    using AccT = Acc<T>;
+    const GeomT<T>& G;
+    EvalLds<T>& L;
+    T cax, cay, cbx, cby;   // crystal axes
+    int npd, only_self;
+    LutDev lut;             // MODE_LUT only
+    long long n_tp, n_pp;   // statistics: tree pairs within cutoff, piece pairs clipped
+
+    __device__ Evaluator(const GeomT<T>& g, EvalLds<T>& l) : G(g), L(l), n_tp(0), n_pp(0) {}
+
+    // Relative vector from tree i to the (shifted) tree j of a tree-pair entry.
+    __device__ inline void rel(uint32_t e, int& i, int& j, T& dx, T& dy, bool& self) const {
+        i = e & 0xfff;
+        j = (e >> 12) & 0xfff;
+        dx = L.x[j] - L.x[i];
+        dy = L.y[j] - L.y[i];
+        self = i == j;
+        if (CRYSTAL) {
+            const int nc = 2 * npd + 1;
+            const int cell = (int)(e >> 24);
+            const int cx = cell / nc - npd, cy = cell % nc - npd;
+            dx += T(cx) * cax + T(cy) * cbx;
+            dy += T(cx) * cay + T(cy) * cby;
+        }
+    }
+
+    __device__ inline void flush_pp(int n) {
+        for (int q = threadIdx.x; q < n; q += BLOCK) {
+            const uint32_t e = L.pp[q];
+            const uint32_t te = L.tp[e & 0xffff];
+            const int pi = (e >> 16) & 0xf, pj = (e >> 20) & 0xf;
+            int i, j;
+            T dx, dy;
+            bool self;
+            rel(te, i, j, dx, dy, self);
+            T ax[MAXV], ay[MAXV], bx[MAXV], by[MAXV];
+            const int oa = (i * L.P + pi) * MAXV, ob = (j * L.P + pj) * MAXV;
+#pragma unroll
+            for (int v = 0; v < MAXV; ++v) {
+                ax[v] = L.vx[oa + v];
+                ay[v] = L.vy[oa + v];
+                bx[v] = L.vx[ob + v] + dx;
+                by[v] = L.vy[ob + v] + dy;
+            }
+            T gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0};
+            T c;
+            if (MODE == MODE_AREA) {
+                c = area_pair<T, GRAD>(ax, ay, bx, by, dx, dy, gi, gj);
+            } else {
+                const T sep = sat_pair<T, GRAD>(ax, ay, bx, by, dx, dy, gi, gj);
+                c = sep * sep;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { gi[k] *= T(2) * sep; gj[k] *= T(2) * sep; }
+            }
+            // Self images (periodic only) count twice when only they are summed and carry
+            // no gradient, as core/pack_cuda.py:349,452-462.
+            if (CRYSTAL && self && only_self) c *= T(2);
+            AccT::add(L.cost, c);
+            if (GRAD && !self) {
+                AccT::add(&L.g[3 * i + 0], gi[0]);
+                AccT::add(&L.g[3 * i + 1], gi[1]);
+                AccT::add(&L.g[3 * i + 2], gi[2]);
+                AccT::add(&L.g[3 * j + 0], gj[0]);
+                AccT::add(&L.g[3 * j + 1], gj[1]);
+                AccT::add(&L.g[3 * j + 2], gj[2]);
+            }
+        }
+        n_pp += n;
+        __syncthreads();
+    }
+
+    // Table lookups for the tree pairs of the list (core/pack_cuda_lut.py:420-547):
+    // relative pose of j in i's frame, value (optionally max(0,v)^2) and gradient to
+    // both trees.
+    __device__ inline void lut_tp(int ntp) {
+        for (int q = threadIdx.x; q < ntp; q += BLOCK) {
+            int i, j;
+            T dx, dy;
+            bool self;
+            rel(L.tp[q], i, j, dx, dy, self);
+            const float cr = L.c[i], sr = L.s[i];
+            const float dxl = cr * dx + sr * dy, dyl = -sr * dx + cr * dy;
+            const float dth = wrap_pi(L.t[j] - L.t[i]);
+            float gl[3];
+            float v = lut_lookup(lut, dxl, dyl, dth, gl);
+            if (lut.quad) {
+                const float k = v > 0.f ? 2.f * v : 0.f;
+                gl[0] *= k; gl[1] *= k; gl[2] *= k;
+                v = v > 0.f ? v * v : 0.f;
+            }
+            AccT::add(L.cost, v);
+            if (GRAD) {
+                AccT::add(&L.g[3 * i + 0], -gl[0] * cr + gl[1] * sr);
+                AccT::add(&L.g[3 * i + 1], -gl[0] * sr - gl[1] * cr);
+                AccT::add(&L.g[3 * i + 2], gl[0] * dyl - gl[1] * dxl - gl[2]);
+                AccT::add(&L.g[3 * j + 0], gl[0] * cr - gl[1] * sr);
+                AccT::add(&L.g[3 * j + 1], gl[0] * sr + gl[1] * cr);
+                AccT::add(&L.g[3 * j + 2], gl[2]);
+            }
+        }
+        n_pp += ntp;
+        __syncthreads();
+    }
+
+    // Expand the tree-pair list into piece pairs (circle test), clipping as the
+    // piece-pair list fills.
+    __device__ inline void flush_tp(int ntp) {
+        if (MODE == MODE_LUT) {
+            lut_tp(ntp);
+            return;
+        }
+        const int PP = L.P * L.P;
+        const int nitems = ntp * PP;
+        int npp = 0;
+        for (int base = 0; base < nitems; base += BLOCK) {
+            const int u = base + threadIdx.x;
+            bool pred = false;
+            uint32_t entry = 0;
+            if (u < nitems) {
+                const int q = u / PP;
+                const int r = u - q * PP;
+                const int pi = r / L.P, pj = r - pi * L.P;
+                int i, j;
+                T dx, dy;
+                bool self;
+                rel(L.tp[q], i, j, dx, dy, self);
+                const T cxa = L.c[i] * G.ccx[pi] - L.s[i] * G.ccy[pi];
+                const T cya = L.s[i] * G.ccx[pi] + L.c[i] * G.ccy[pi];
+                const T cxb = L.c[j] * G.ccx[pj] - L.s[j] * G.ccy[pj] + dx;
+                const T cyb = L.s[j] * G.ccx[pj] + L.c[j] * G.ccy[pj] + dy;
+                const T rr = G.cr[pi] + G.cr[pj];
+                const T ddx = cxb - cxa, ddy = cyb - cya;
+                pred = ddx * ddx + ddy * ddy <= rr * rr;
+                entry = (uint32_t)q | ((uint32_t)pi << 16) | ((uint32_t)pj << 20);
+            }
+            int total;
+            const int slot = block_append<BLOCK>(pred, L.wcnt, total);
+            if (npp + total > PP_CAP) {
+                flush_pp(npp);
+                npp = 0;
+            }
+            if (pred) L.pp[npp + slot] = entry;
+            npp += total;
+            __syncthreads();
+        }
+        flush_pp(npp);
+    }
+
+    // Enumerate candidate tree pairs and process them.
+    __device__ inline void run() {
+        const int N = L.N;
+        int ntp = 0;
+        if (!CRYSTAL) {
+            // Round-robin order: pairs (i, (i + d) mod N), d = 1..D, then d = N/2 for
+            // i < N/2 when N is even; every unordered pair exactly once.
+            const int D = (N - 1) / 2;
+            const long long nfull = (long long)N * D;
+            const long long ncand = (long long)N * (N - 1) / 2;
+            for (long long base = 0; base < ncand; base += BLOCK) {
+                const long long k = base + threadIdx.x;
+                bool pred = false;
+                uint32_t entry = 0;
+                if (k < ncand) {
+                    int i, d;
+                    if (k < nfull) {
+                        d = (int)(k / N) + 1;
+                        i = (int)(k - (long long)(d - 1) * N);
+                    } else {
+                        i = (int)(k - nfull);
+                        d = N / 2;
+                    }
+                    int j = i + d;
+                    j = j >= N ? j - N : j;
+                    const int a = i < j ? i : j, b = i < j ? j : i;
+                    const T dx = L.x[b] - L.x[a], dy = L.y[b] - L.y[a];
+                    pred = dx * dx + dy * dy <= G.cutoff_sq;
+                    entry = (uint32_t)a | ((uint32_t)b << 12);
+                }
+                int total;
+                const int slot = block_append<BLOCK>(pred, L.wcnt, total);
+                if (ntp + total > TP_CAP) {
+                    flush_tp(ntp);
+                    ntp = 0;
+                }
+                if (pred) L.tp[ntp + slot] = entry;
+                ntp += total;
+                n_tp += total;
+                __syncthreads();
+            }
+        } else {
+            // Pairs i <= j times lattice cells; a tree's own images only for the positive
+            // half of the cells (image s and -s have equal overlap).
+            const int nc = 2 * npd + 1, ncell = nc * nc, center = (ncell - 1) / 2;
+            const long long npair = (long long)N * (N + 1) / 2;
+            const long long ncand = npair * ncell;
+            for (long long base = 0; base < ncand; base += BLOCK) {
+                const long long k = base + threadIdx.x;
+                bool pred = false;
+                uint32_t entry = 0;
+                if (k < ncand) {
+                    const int cell = (int)(k % ncell);
+                    long long q = k / ncell;
+                    int i = 0;
+                    while (q >= N - i) { q -= N - i; ++i; }
+                    const int j = i + (int)q;
+                    const bool self = i == j;
+                    bool ok = self ? cell > center : !only_self;
+                    const uint32_t e = (uint32_t)i | ((uint32_t)j << 12) | ((uint32_t)cell << 24);
+                    int ii, jj;
+                    T dx, dy;
+                    bool sf;
+                    rel(e, ii, jj, dx, dy, sf);
+                    pred = ok && (dx * dx + dy * dy <= G.cutoff_sq);
+                    entry = e;
+                }
+                int total;
+                const int slot = block_append<BLOCK>(pred, L.wcnt, total);
+                if (ntp + total > TP_CAP) {
+                    flush_tp(ntp);
+                    ntp = 0;
+                }
+                if (pred) L.tp[ntp + slot] = entry;
+                ntp += total;
+                n_tp += total;
+                __syncthreads();
+            }
+        }
+        flush_tp(ntp);
+    }
+};
+
+// Stage poses (already in L.x/y/t) into sincos and rotated piece vertices; clear the
+// accumulators.  Ends with a barrier.
+template <typename T, int BLOCK, bool VERTS = true>
+__device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
+    const int N = L.N, P = L.P;
+    for (int i = threadIdx.x; i < N; i += BLOCK) {
+        T sn, cs;
+        sincos_t(L.t[i], &sn, &cs);
+        L.c[i] = cs;
+        L.s[i] = sn;
+    }
+    for (int q = threadIdx.x; q < 3 * N + 1; q += BLOCK) L.g[q] = Acc<T>::zero();
+    __syncthreads();
+    if (!VERTS) return;
+    const int nv = N * P * MAXV;
+    for (int q = threadIdx.x; q < nv; q += BLOCK) {
+        const int i = q / (P * MAXV);
+        const int r = q - i * P * MAXV;
+        const T lx = G.px[r], ly = G.py[r];
+        L.vx[q] = L.c[i] * lx - L.s[i] * ly;
+        L.vy[q] = L.s[i] * lx + L.c[i] * ly;
+    }
+    __syncthreads();
+}
+
+// Boundary term of one tree (core/pack_cuda.py:550-642): max over outline vertices of
+// the squared distance outside the square h = [size, ox, oy]; gradients w.r.t. the pose
+// and h at the arg-max vertex (strict >, first vertex on ties, 0 if inside).
+template <typename T>
+__device__ inline T boundary_tree(const GeomT<T>& G, T x, T y, T c, T s, T half, T hx, T hy,
+                                  T (&g)[3], T (&gh)[3]) {
+    T best = T(0);
+    int bi = 0;
+    for (int v = 0; v < G.n_outline; ++v) {
+        const T lx = G.ox[v], ly = G.oy[v];
+        const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
+        const T ax = fabs(vx), ay = fabs(vy);
+        const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);
+        const T d2 = dx * dx + dy * dy;
+        if (d2 > best) { best = d2; bi = v; }
+    }
+    const T lx = G.ox[bi], ly = G.oy[bi];
+    const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
+    const T ax = fabs(vx), ay = fabs(vy);
+    const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);
+    const T gvx = dx > T(0) ? T(2) * dx * (vx >= T(0) ? T(1) : T(-1)) : T(0);
+    const T gvy = dy > T(0) ? T(2) * dy * (vy >= T(0) ? T(1) : T(-1)) : T(0);
+    g[0] = gvx;
+    g[1] = gvy;
+    g[2] = gvx * (-s * lx - c * ly) + gvy * (c * lx - s * ly);
+    gh[0] = -(dx + dy);
+    gh[1] = -gvx;
+    gh[2] = -gvy;
+    return best;
+}
+
+}  // namespace pk

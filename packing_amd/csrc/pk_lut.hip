@@ -1,0 +1,196 @@
+// Pose-space lookup table: handle, on-device builder (exact clipping of the polygon at the
+// origin against the polygon at every grid pose) and the table-based overlap cost.
+#include <cstring>
+#include <string>
+
+#include "pk_eval.h"
+#include "packing.h"
+
+namespace pk {
+
+struct Lut {
+    LutDev d;
+};
+
+LutDev make_lut_dev(const void* h) { return reinterpret_cast<const Lut*>(h)->d; }
+
+// One thread per grid node: area of (polygon at 0, theta 0) ∩ (polygon at (x, y, theta)).
+// Equals what core/pack_cost.py:333-366 tabulates through the exact kernel.
+__global__ __launch_bounds__(256) void k_lut_build(const GeomT<float>* __restrict__ geom,
+                                                   const float* __restrict__ X, const float* __restrict__ Y,
+                                                   const float* __restrict__ Th, int nx, int ny, int nt,
+                                                   float* __restrict__ out) {
+    __shared__ GeomT<float> G;
+    load_geom(G, geom);
+    __syncthreads();
+    const long long total = (long long)nx * ny * nt;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int it = (int)(q % nt);
+        const long long r = q / nt;
+        const int iy = (int)(r % ny), ix = (int)(r / ny);
+        const float dx = X[ix], dy = Y[iy];
+        float s, c;
+        sincosf(Th[it], &s, &c);
+        float acc = 0.f;
+        if (dx * dx + dy * dy <= G.cutoff_sq) {
+            for (int pi = 0; pi < G.P; ++pi) {
+                float ax[MAXV], ay[MAXV];
+#pragma unroll
+                for (int v = 0; v < MAXV; ++v) {
+                    ax[v] = G.px[pi * MAXV + v];
+                    ay[v] = G.py[pi * MAXV + v];
+                }
+                for (int pj = 0; pj < G.P; ++pj) {
+                    const float cbx = c * G.ccx[pj] - s * G.ccy[pj] + dx - G.ccx[pi];
+                    const float cby = s * G.ccx[pj] + c * G.ccy[pj] + dy - G.ccy[pi];
+                    const float rr = G.cr[pi] + G.cr[pj];
+                    if (cbx * cbx + cby * cby > rr * rr) continue;
+                    float bx[MAXV], by[MAXV];
+#pragma unroll
+                    for (int v = 0; v < MAXV; ++v) {
+                        bx[v] = c * G.px[pj * MAXV + v] - s * G.py[pj * MAXV + v] + dx;
+                        by[v] = s * G.px[pj * MAXV + v] + c * G.py[pj * MAXV + v] + dy;
+                    }
+                    float gi[3], gj[3];
+                    acc += area_pair<float, false>(ax, ay, bx, by, dx, dy, gi, gj);
+                }
+            }
+        }
+        out[q] = acc;
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_lut_overlap(const float* __restrict__ xyt,
+                                                       const int* __restrict__ inst_off, int n_trees,
+                                                       int n_ens, const GeomT<float>* __restrict__ geom,
+                                                       LutDev lut, float* __restrict__ out_cost,
+                                                       float* __restrict__ out_grads,
+                                                       unsigned long long* __restrict__ stats) {
+    extern __shared__ __align__(16) char smem[];
+    __shared__ GeomT<float> G;
+    const int e = blockIdx.x;
+    if (e >= n_ens) return;
+    int off, N;
+    if (inst_off) {
+        off = inst_off[e];
+        N = inst_off[e + 1] - off;
+    } else {
+        off = e * n_trees;
+        N = n_trees;
+    }
+    load_geom(G, geom);
+    __syncthreads();
+    EvalLds<float> L = carve_eval_lds<float>(smem, N, 0);
+    for (int i = threadIdx.x; i < N; i += BLOCK) {
+        L.x[i] = xyt[3 * (off + i) + 0];
+        L.y[i] = xyt[3 * (off + i) + 1];
+        L.t[i] = xyt[3 * (off + i) + 2];
+    }
+    __syncthreads();
+    stage_instance<float, BLOCK, false>(G, L);
+    Evaluator<float, MODE_LUT, true, false, BLOCK> ev(G, L);
+    ev.cax = ev.cay = ev.cbx = ev.cby = 0.f;
+    ev.npd = 0;
+    ev.only_self = 0;
+    ev.lut = lut;
+    ev.run();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out_cost[e] = Acc<float>::get(*L.cost);
+        if (stats) {
+            atomicAdd(&stats[0], (unsigned long long)ev.n_tp);
+            atomicAdd(&stats[1], (unsigned long long)ev.n_pp);
+        }
+    }
+    if (out_grads)
+        for (int q = threadIdx.x; q < 3 * N; q += BLOCK) out_grads[3 * off + q] = Acc<float>::get(L.g[q]);
+}
+
+}  // namespace pk
+
+using namespace pk;
+
+extern "C" {
+
+int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, double x_max, double y_min,
+                  double y_max, double t_min, double t_max, int quad, int mirror, pk_lut_t* out) {
+    PK_REQUIRE(out != nullptr && vals != nullptr, "pk_lut_create: NULL argument");
+    PK_REQUIRE(nx >= 2 && ny >= 2 && nt >= 2, "pk_lut_create: every axis needs >= 2 nodes");
+    PK_REQUIRE(x_max > x_min && y_max > y_min && t_max > t_min, "pk_lut_create: empty range");
+    Lut* L = new Lut();
+    L->d.vals = vals;
+    L->d.nx = nx;
+    L->d.ny = ny;
+    L->d.nt = nt;
+    L->d.x_min = (float)x_min;
+    L->d.x_max = (float)x_max;
+    L->d.x_lo = mirror ? (float)-x_max : (float)x_min;
+    L->d.y_min = (float)y_min;
+    L->d.y_max = (float)y_max;
+    L->d.t_min = (float)t_min;
+    L->d.t_max = (float)t_max;
+    L->d.sx = (float)((nx - 1) / (x_max - x_min));
+    L->d.sy = (float)((ny - 1) / (y_max - y_min));
+    L->d.st = (float)((nt - 1) / (t_max - t_min));
+    L->d.quad = quad ? 1 : 0;
+    L->d.mirror = mirror ? 1 : 0;
+    *out = reinterpret_cast<pk_lut_t>(L);
+    return PK_OK;
+}
+
+int pk_lut_destroy(pk_lut_t h) {
+    delete reinterpret_cast<Lut*>(h);
+    return PK_OK;
+}
+
+int pk_lut_build(pk_geometry_t h, const float* X, int nx, const float* Y, int ny, const float* T, int nt,
+                 float* out_vals, void* stream) {
+    const Geometry* g = reinterpret_cast<const Geometry*>(h);
+    PK_REQUIRE(g && X && Y && T && out_vals, "pk_lut_build: NULL argument");
+    PK_REQUIRE(nx >= 1 && ny >= 1 && nt >= 1, "pk_lut_build: empty grid");
+    const long long total = (long long)nx * ny * nt;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_lut_build, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), g->df, X, Y, T, nx, ny, nt, out_vals);
+    PK_CHECK_HIP(hipGetLastError());
+    return PK_OK;
+}
+
+int pk_lut_overlap(pk_geometry_t h, pk_lut_t lut, const float* xyt, const int32_t* inst_off, int n_ens,
+                   int n_trees, float* out_cost, float* out_grads, int block,
+                   unsigned long long* stats, void* stream) {
+    const Geometry* g = reinterpret_cast<const Geometry*>(h);
+    PK_REQUIRE(g && lut, "pk_lut_overlap: NULL geometry or table");
+    PK_REQUIRE(n_ens >= 0 && n_trees >= 0 && n_trees <= 4096, "pk_lut_overlap: bad size");
+    if (n_ens == 0) return PK_OK;
+    PK_REQUIRE(xyt && out_cost, "pk_lut_overlap: xyt and out_cost are required");
+    const LutDev d = reinterpret_cast<const Lut*>(lut)->d;
+    const size_t lds = eval_lds_bytes<float>(n_trees, 0);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int blk = block ? block : 256;
+#define PK_LO(B)                                                                                   \
+    {                                                                                              \
+        auto kern = k_lut_overlap<B>;                                                              \
+        if (lds > 60000)                                                                           \
+            PK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                  \
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        hipLaunchKernelGGL(kern, dim3(n_ens), dim3(B), lds, st, xyt, inst_off, n_trees, n_ens,      \
+                           g->df, d, out_cost, out_grads, stats);                                  \
+        PK_CHECK_HIP(hipGetLastError());                                                           \
+        return PK_OK;                                                                              \
+    }
+    switch (blk) {
+        case 64: PK_LO(64)
+        case 128: PK_LO(128)
+        case 256: PK_LO(256)
+        default:
+            set_error("pk_lut_overlap: block must be 64, 128 or 256");
+            return PK_EINVAL;
+    }
+#undef PK_LO
+}
+
+}  // extern "C"

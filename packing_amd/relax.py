@@ -1,0 +1,202 @@
+"""Batched relaxation engine over the fused HIP kernel (pk_relax_step).
+
+One workgroup relaxes one instance: cost evaluation and the L-BFGS update run inside
+the kernel, so a ragged batch of thousands of instances advances by ``iters``
+iterations per launch (``iters = max_iter`` relaxes the whole batch in one launch).
+Each instance follows core/lbfgs_torch_parallel.py:411-729 (no line search) on its
+own, with the objective of core/pack_dynamics.py:93-123 (overlap + boundary [+ area],
+h fixed by default), so its result is the same alone, in any batch and with any
+workgroup size.
+
+Paths:
+  * ``RelaxBatch.run()``            batched: all instances per launch, chunked launches;
+  * ``RelaxBatch.run_plain()``      plain: one launch per instance per iteration (the
+                                    baseline; bitwise the same results);
+  * ``RelaxBatch.capture_graph()``  the chunk launch captured as a HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .context import device_geometry, get_geometry
+
+MODES = {"area": N.PK_MODE_AREA, "sep": N.PK_MODE_SEP, "lut": N.PK_MODE_LUT}
+
+
+@dataclass
+class RelaxOptions:
+    mode: str = "area"
+    w_overlap: float = 1.0
+    w_boundary: float = 1.0
+    w_area: float = 0.0
+    fixed_h: bool = True
+    history: int = 8
+    max_iter: int = 500
+    max_eval: int | None = None
+    lr: float = 1.0
+    max_step: float | None = 0.1
+    tol_grad: float = 0.0
+    tol_change: float = 0.0
+    tol_rel: float = 0.0
+    stop_on_increase: bool = False
+    block: int = 0          # 0: chosen by the tile table (autotune.py)
+    lut: object = None      # pack_cuda_lut.LookupTable for mode "lut"
+
+
+def state_size(n_trees) -> np.ndarray:
+    return 3 * np.asarray(n_trees, dtype=np.int64) + 3
+
+
+class RelaxBatch:
+    """Device-resident state of a ragged batch of relaxations.
+
+    xyt: list of (N_i, 3) arrays/tensors, or a (M, N, 3) array; h: (M, 3) boxes.
+    """
+
+    def __init__(self, xyt, h, opts: RelaxOptions | None = None, geometry=None, device="cuda",
+                 capacity_trees: int | None = None):
+        self.opts = opts or RelaxOptions()
+        self.device = torch.device(device)
+        self.geometry = geometry or get_geometry()
+        if isinstance(xyt, (np.ndarray, torch.Tensor)) and np.ndim(xyt) == 3:
+            xyt = [xyt[i] for i in range(xyt.shape[0])]
+        sizes = np.array([int(a.shape[0]) for a in xyt], dtype=np.int64)
+        self.M = len(sizes)
+        self.sizes = sizes
+        n = state_size(sizes)
+        xoff = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+        if n.sum() >= 2 ** 31:
+            raise ValueError("batch too large for int32 state offsets")
+        self.n_state = int(n.sum())
+        self.xoff_np = xoff
+        h = np.asarray(h.cpu() if isinstance(h, torch.Tensor) else h, dtype=np.float32).reshape(self.M, 3)
+        xs = np.empty(self.n_state, dtype=np.float32)
+        for i, a in enumerate(xyt):
+            a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+            o = xoff[i]
+            xs[o:o + 3 * sizes[i]] = a.astype(np.float32).reshape(-1)
+            xs[o + 3 * sizes[i]:o + 3 * sizes[i] + 3] = h[i]
+        dev = self.device
+        m = self.opts.history
+        self.x = torch.tensor(xs, device=dev)
+        self.g = torch.zeros_like(self.x)
+        self.s_hist = torch.zeros(m * self.n_state, dtype=torch.float32, device=dev)
+        self.y_hist = torch.zeros_like(self.s_hist)
+        self.state = torch.zeros(self.M * N.PK_LBFGS_SLOT_BYTES, dtype=torch.uint8, device=dev)
+        self.xoff = torch.tensor(xoff.astype(np.int32), device=dev)
+        self.ntree = torch.tensor(sizes.astype(np.int32), device=dev)
+        # longest-processing-time first: big instances start first
+        order = np.argsort(-sizes, kind="stable").astype(np.int32)
+        self.order = torch.tensor(order, device=dev)
+        self.status = torch.zeros(self.M, dtype=torch.int32, device=dev)
+        self.loss = torch.zeros(self.M, dtype=torch.float32, device=dev)
+        self.stats = torch.zeros(2, dtype=torch.int64, device=dev)
+        self.n_tree_max = int(sizes.max()) if self.M else 1
+        self._dg = device_geometry(dev, self.geometry)
+        self._graph = None
+
+    # ------------------------------------------------------------------ params
+    def _params(self, iters, order=None, n_launch=None, n_tree_max=None, block=None) -> N.RelaxParams:
+        o = self.opts
+        p = N.RelaxParams()
+        p.n_slots = self.M
+        p.xoff = self.xoff.data_ptr()
+        p.ntree = self.ntree.data_ptr()
+        p.x = self.x.data_ptr()
+        p.g = self.g.data_ptr()
+        p.s_hist = self.s_hist.data_ptr()
+        p.y_hist = self.y_hist.data_ptr()
+        p.state = self.state.data_ptr()
+        p.order = (self.order if order is None else order).data_ptr()
+        p.n_launch = self.M if n_launch is None else n_launch
+        p.n_tree_max = self.n_tree_max if n_tree_max is None else n_tree_max
+        p.mode = MODES[o.mode]
+        p.w_overlap, p.w_boundary, p.w_area = o.w_overlap, o.w_boundary, o.w_area
+        p.fixed_h = int(o.fixed_h)
+        if o.mode == "lut":
+            if o.lut is None:
+                raise ValueError("mode 'lut' needs opts.lut")
+            p.lut = o.lut.handle(self.device)
+        p.history = o.history
+        p.max_iter = o.max_iter
+        p.max_eval = 0 if o.max_eval is None else o.max_eval
+        p.lr = o.lr
+        p.max_step = 0.0 if o.max_step is None else o.max_step
+        p.tol_grad, p.tol_change, p.tol_rel = o.tol_grad, o.tol_change, o.tol_rel
+        p.stop_on_increase = int(o.stop_on_increase)
+        p.iters = int(iters)
+        p.block = int(self.opts.block if block is None else block) or self._auto_block()
+        p.status_out = self.status.data_ptr()
+        p.loss_out = self.loss.data_ptr()
+        p.stats = self.stats.data_ptr()
+        return p
+
+    def _auto_block(self) -> int:
+        from .autotune import choose_block
+        return choose_block(self.opts.mode, self.n_tree_max, self.geometry.n_pieces)
+
+    def launch(self, iters: int, stream=None, **kw) -> None:
+        p = self._params(iters, **kw)
+        N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), N.stream_handle(stream)),
+                "pk_relax_step")
+
+    # ------------------------------------------------------------------ paths
+    def run(self, iters_per_launch: int | None = None, stream=None) -> None:
+        """Batched path: every instance in each launch, until max_iter."""
+        k = iters_per_launch or self.opts.max_iter
+        n_launch = -(-self.opts.max_iter // k)
+        for _ in range(n_launch):
+            self.launch(k, stream=stream)
+
+    def run_plain(self, instances=None, stream=None) -> None:
+        """Baseline path: one launch per instance per iteration."""
+        idx = range(self.M) if instances is None else instances
+        one = torch.empty(1, dtype=torch.int32, device=self.device)
+        for i in idx:
+            one.fill_(int(i))
+            for _ in range(self.opts.max_iter):
+                self.launch(1, order=one, n_launch=1, n_tree_max=int(self.sizes[i]), stream=stream)
+
+    def capture_graph(self, iters_per_launch: int, stream=None):
+        """Capture one chunk launch (all instances, iters_per_launch iterations) as a graph;
+        replaying it advances the batch like run(iters_per_launch) does per launch."""
+        s = stream or torch.cuda.Stream(device=self.device)
+        g = torch.cuda.CUDAGraph()
+        # warm the kernel attribute / lazy state outside the capture on the same stream
+        with torch.cuda.stream(s):
+            p = self._params(iters_per_launch)
+            p.iters = 0
+            N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), N.stream_handle(s)), "warmup")
+            torch.cuda.current_stream().synchronize()
+            with torch.cuda.graph(g, stream=s):
+                self.launch(iters_per_launch, stream=s)
+        self._graph = g
+        return g
+
+    # ------------------------------------------------------------------ results
+    def done(self) -> torch.Tensor:
+        return self.status.bool()
+
+    def xyt(self, i: int) -> torch.Tensor:
+        o = int(self.xoff_np[i])
+        n = int(self.sizes[i])
+        return self.x[o:o + 3 * n].view(n, 3)
+
+    def xyt_list(self):
+        return [self.xyt(i) for i in range(self.M)]
+
+    def xyt_flat(self) -> torch.Tensor:
+        """(sum N, 3) poses of all instances back to back."""
+        idx = np.concatenate([np.arange(o, o + 3 * n) for o, n in zip(self.xoff_np, self.sizes)])
+        return self.x[torch.tensor(idx, device=self.device)].view(-1, 3)
+
+    def slot_records(self) -> np.ndarray:
+        """Per-instance (n_iter, n_eval, hist_len, status, reason) read back from the device."""
+        raw = self.state.view(self.M, N.PK_LBFGS_SLOT_BYTES).cpu().numpy()
+        ints = raw[:, 152:184].view(np.int32)
+        return ints[:, [0, 1, 2, 4, 5]]

@@ -1,0 +1,29 @@
+"""The C-ABI library loads and exports every symbol include/packing.h declares (no GPU)."""
+import os
+import re
+
+from packing_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "packing.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pk_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    L = _native.lib()
+    names = declared()
+    assert names, "no declarations parsed"
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert sorted(_native.exported_symbols()) == declared()
+
+
+def test_version():
+    assert _native.lib().pk_version() == 1
