@@ -153,6 +153,16 @@ typedef struct {
 } pk_relax_params;
 
 int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
+
+/*
+ * FP64 zero-overlap check of final packings: out_pairs[e] = number of tree pairs of
+ * instance e whose interiors intersect (touching allowed), with poses, rotations and
+ * projections in double.  Semantics of the Kaggle metric's collision test
+ * (core/pack_metric.py:130-141) and of pack_io.legalize's acceptance
+ * (core/pack_io.py:95-101); the reference checks it on the CPU with shapely.
+ */
+int pk_feasible(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst_off, int n_ens,
+                int n_trees, int32_t* out_pairs, void* stream);
 int pk_relax_slot_bytes(void);
 int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max);
 

@@ -16,6 +16,7 @@ SOURCES = [
     "pk_overlap.hip",
     "pk_lut.hip",
     "pk_relax.hip",
+    "pk_feasible.hip",
 ]
 
 ARCH = os.environ.get("PK_OFFLOAD_ARCH", "gfx950")

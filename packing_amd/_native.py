@@ -42,6 +42,7 @@ _SIGS = {
     "pk_relax_step": (i32, [vp, vp, vp]),
     "pk_relax_slot_bytes": (i32, []),
     "pk_relax_lds_bytes": (i32, [vp, i32, i32]),
+    "pk_feasible": (i32, [vp, i32, vp, vp, i32, i32, vp, vp]),
 }
 
 

@@ -99,6 +99,15 @@ class RelaxBatch:
         self.n_tree_max = int(sizes.max()) if self.M else 1
         self._dg = device_geometry(dev, self.geometry)
         self._graph = None
+        self.x0 = self.x.clone()
+
+    def reset(self) -> None:
+        """Back to the initial poses with fresh optimizer state (device-side copies, no
+        allocation; history needs no clearing since a fresh slot never reads it)."""
+        torch.mul(self.x0, 1.0, out=self.x)   # a kernel, so it can sit in a captured graph
+        self.state.zero_()
+        self.status.zero_()
+        self.stats.zero_()
 
     # ------------------------------------------------------------------ params
     def _params(self, iters, order=None, n_launch=None, n_tree_max=None, block=None) -> N.RelaxParams:
@@ -173,7 +182,7 @@ class RelaxBatch:
             p.iters = 0
             N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), N.stream_handle(s)), "warmup")
             torch.cuda.current_stream().synchronize()
-            with torch.cuda.graph(g, stream=s):
+            with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
                 self.launch(iters_per_launch, stream=s)
         self._graph = g
         return g
