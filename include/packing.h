@@ -28,6 +28,7 @@ extern "C" {
 #define PK_OK 0
 #define PK_F32 0
 #define PK_F64 1
+#define PK_I32 2
 #define PK_MODE_AREA 0
 #define PK_MODE_SEP 1
 #define PK_MODE_LUT 2
@@ -163,6 +164,28 @@ int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
  */
 int pk_feasible(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst_off, int n_ens,
                 int n_trees, int32_t* out_pairs, void* stream);
+
+/*
+ * Batched Hungarian: B square N x N cost matrices (dtype PK_F32 or PK_I32, N <= 256),
+ * row_match (B, N) int32.  Replaces lap_batch.solve_lap_batch with the 'hungarian'
+ * algorithm (core/lap_batch.py:457-534, kernels hungarian_kernel :62-165 and
+ * compute_costs :173-196).  Same scan order and strict-< tie-breaks as the reference, so
+ * row_match is bit-identical; float matrices use the reference's float32 arithmetic,
+ * int32 matrices exact int64 potentials.  total = float32 sequential row-order sum
+ * (PK_F32) or exact int64 sum (PK_I32); total_f32 (nullable) = float32 sequential sum in
+ * both cases.
+ */
+int pk_lap_hungarian(int dtype, const void* cost, int B, int N, int32_t* row_match, void* total,
+                     float* total_f32, void* stream);
+
+/*
+ * Diversity shortcut: out[p * n_ref + r] = sum over population trees of the distance in
+ * (x, y, wrapped theta) to the nearest tree of reference r after its optional mirror and
+ * rotation (cos_a, sin_a).  Replaces lap_batch.compute_diversity_shortcut_kernel and its
+ * kernel (core/lap_batch.py:216-331,537-608).  float32.
+ */
+int pk_diversity(const float* pop, const float* ref, int n_pop, int n_ref, int n_trees, float cos_a,
+                 float sin_a, int mirror, float* out, void* stream);
 int pk_relax_slot_bytes(void);
 int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max);
 

@@ -17,6 +17,7 @@ SOURCES = [
     "pk_lut.hip",
     "pk_relax.hip",
     "pk_feasible.hip",
+    "pk_lap.hip",
 ]
 
 ARCH = os.environ.get("PK_OFFLOAD_ARCH", "gfx950")

@@ -12,7 +12,7 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpacking.so")
 
-PK_F32, PK_F64 = 0, 1
+PK_F32, PK_F64, PK_I32 = 0, 1, 2
 PK_MODE_AREA, PK_MODE_SEP, PK_MODE_LUT = 0, 1, 2
 PK_MAX_HISTORY = 16
 PK_LBFGS_SLOT_BYTES = 184
@@ -43,6 +43,8 @@ _SIGS = {
     "pk_relax_slot_bytes": (i32, []),
     "pk_relax_lds_bytes": (i32, [vp, i32, i32]),
     "pk_feasible": (i32, [vp, i32, vp, vp, i32, i32, vp, vp]),
+    "pk_lap_hungarian": (i32, [i32, vp, i32, i32, vp, vp, vp, vp]),
+    "pk_diversity": (i32, [vp, vp, i32, i32, i32, C.c_float, C.c_float, i32, vp, vp]),
 }
 
 

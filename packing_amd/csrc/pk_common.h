@@ -91,6 +91,8 @@ struct Acc<double> {
     __device__ static inline type zero() { return 0.0; }
 };
 
+__host__ __device__ inline size_t align16(size_t v) { return (v + 15) / 16 * 16; }
+
 __host__ __device__ inline float fast_rcp(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_rcpf(x);
