@@ -44,7 +44,7 @@ def test_diversity_closed_form(oracle_mod):
     rng = np.random.default_rng(1)
     pop = rng.uniform(-2, 2, (3, 9, 3)).astype(np.float32)
     out = oracle_mod.diversity(pop, pop, 1.0, 0.0, False)
-    assert np.allclose(np.diag(out), 0, atol=1e-6)
+    assert np.allclose(np.diag(out), 0, atol=1e-5)   # theta passes through float wrap
     one = rng.uniform(-2, 2, (3, 1, 3)).astype(np.float32)
     shifted = one.copy()
     shifted[..., 0] += 3.0
