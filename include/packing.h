@@ -34,6 +34,7 @@ extern "C" {
 #define PK_MODE_LUT 2
 #define PK_MAX_HISTORY 16
 #define PK_LBFGS_SLOT_BYTES 184
+#define PK_LBFGS_SYS_BYTES 48
 
 typedef struct pk_geometry_opaque* pk_geometry_t;
 typedef struct pk_lut_opaque* pk_lut_t;
@@ -154,6 +155,39 @@ typedef struct {
 } pk_relax_params;
 
 int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
+
+/*
+ * Batched L-BFGS for a caller-evaluated objective: the drop-in for
+ * lbfgs_torch_parallel.lbfgs without line search (core/lbfgs_torch_parallel.py:411-729),
+ * batch-coupled exactly as the reference (the run stops when no system is active; until
+ * then inactive systems keep stepping).  Protocol, with x, g, loss holding the iterate,
+ * its gradient and loss:  evaluate f(x0) -> pk_lbfgs_begin;  for it = 1..max_iter:
+ * pk_lbfgs_direction(it), pk_lbfgs_step(it), then (it < max_iter) evaluate f(x) into g
+ * and loss.  counters (2 * max_iter + 4 ints, zeroed by the caller) hold the stop state;
+ * counters[0] != 0 once the run has stopped.  Sizes: d, g_prev (M, n); s_hist, y_hist
+ * (M, history, n); ro, al (M, history) double; sys M * PK_LBFGS_SYS_BYTES.
+ */
+typedef struct {
+    int M, n, history, dtype;            /* dtype PK_F32 or PK_F64                      */
+    void* x;
+    void* g;
+    void* loss;
+    void* d;
+    void* g_prev;
+    void* s_hist;
+    void* y_hist;
+    void* ro;
+    void* al;
+    void* sys;
+    int* counters;
+    double lr, tol_grad, tol_change, tol_rel, max_step;   /* max_step <= 0: no clamp     */
+    int max_eval, stop_on_increase;
+} pk_lbfgs_batch;
+
+int pk_lbfgs_sys_bytes(void);
+int pk_lbfgs_begin(const pk_lbfgs_batch* b, void* stream);
+int pk_lbfgs_direction(const pk_lbfgs_batch* b, int it, void* stream);
+int pk_lbfgs_step(const pk_lbfgs_batch* b, int it, void* stream);
 
 /*
  * FP64 zero-overlap check of final packings: out_pairs[e] = number of tree pairs of

@@ -18,6 +18,7 @@ SOURCES = [
     "pk_relax.hip",
     "pk_feasible.hip",
     "pk_lap.hip",
+    "pk_lbfgs.hip",
 ]
 
 ARCH = os.environ.get("PK_OFFLOAD_ARCH", "gfx950")

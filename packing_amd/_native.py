@@ -16,6 +16,7 @@ PK_F32, PK_F64, PK_I32 = 0, 1, 2
 PK_MODE_AREA, PK_MODE_SEP, PK_MODE_LUT = 0, 1, 2
 PK_MAX_HISTORY = 16
 PK_LBFGS_SLOT_BYTES = 184
+PK_LBFGS_SYS_BYTES = 48
 
 _lib = None
 _lock = threading.Lock()
@@ -43,6 +44,10 @@ _SIGS = {
     "pk_relax_slot_bytes": (i32, []),
     "pk_relax_lds_bytes": (i32, [vp, i32, i32]),
     "pk_feasible": (i32, [vp, i32, vp, vp, i32, i32, vp, vp]),
+    "pk_lbfgs_sys_bytes": (i32, []),
+    "pk_lbfgs_begin": (i32, [vp, vp]),
+    "pk_lbfgs_direction": (i32, [vp, i32, vp]),
+    "pk_lbfgs_step": (i32, [vp, i32, vp]),
     "pk_lap_hungarian": (i32, [i32, vp, i32, i32, vp, vp, vp, vp]),
     "pk_diversity": (i32, [vp, vp, i32, i32, i32, C.c_float, C.c_float, i32, vp, vp]),
 }
@@ -59,6 +64,15 @@ class RelaxParams(C.Structure):
         ("tol_change", C.c_float), ("tol_rel", C.c_float), ("stop_on_increase", i32),
         ("iters", i32), ("block", i32), ("status_out", vp), ("loss_out", vp), ("stats", vp),
     ]
+
+
+class LbfgsBatch(C.Structure):
+    """ctypes mirror of pk_lbfgs_batch (include/packing.h)."""
+    _fields_ = [("M", i32), ("n", i32), ("history", i32), ("dtype", i32), ("x", vp), ("g", vp),
+                ("loss", vp), ("d", vp), ("g_prev", vp), ("s_hist", vp), ("y_hist", vp), ("ro", vp),
+                ("al", vp), ("sys", vp), ("counters", vp), ("lr", C.c_double), ("tol_grad", C.c_double),
+                ("tol_change", C.c_double), ("tol_rel", C.c_double), ("max_step", C.c_double),
+                ("max_eval", i32), ("stop_on_increase", i32)]
 
 
 class PackingError(RuntimeError):
