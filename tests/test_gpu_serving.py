@@ -1,0 +1,56 @@
+"""Continuous batching: requests admitted while others run, HBM-budget admission, and
+every served result bitwise equal to relaxing that request alone."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import GEOMS, require_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    require_gpu()
+
+
+def test_served_equals_alone():
+    from packing_amd.context import set_geometry
+    from packing_amd.relax import RelaxBatch, RelaxOptions
+    from packing_amd.serving import RelaxServer, poisson_requests
+    geom = GEOMS["random"]()
+    set_geometry(geom)
+    opts = RelaxOptions(mode="area", max_iter=60, history=8, max_step=0.1)
+    reqs = poisson_requests(40, rate=2000.0, n_lo=10, n_hi=60)
+    srv = RelaxServer(opts, n_slots=16, n_max=60, chunk=20)
+    rep = srv.serve(reqs)
+    assert rep["requests"] == 40 and rep["launches"] > 3
+    # requests that arrived while others were running were admitted later than t=0
+    assert max(r.t_admit for r in reqs) > 0
+    for r in reqs[::7]:
+        b = RelaxBatch([r.xyt], r.h[None], opts, geometry=geom)
+        b.run()
+        assert torch.equal(b.xyt(0), r.result), r.rid
+
+
+def test_budget_limits_concurrency():
+    from packing_amd.context import set_geometry
+    from packing_amd.relax import RelaxOptions
+    from packing_amd.serving import RelaxServer, poisson_requests, slot_bytes
+    set_geometry(GEOMS["tree"]())
+    opts = RelaxOptions(mode="area", max_iter=30, history=4, max_step=0.1)
+    per = slot_bytes(40, 4)
+    srv = RelaxServer(opts, n_slots=64, n_max=40, chunk=10, hbm_budget_bytes=3 * per)
+    assert srv.S == 3
+    seen = []
+    orig = srv._launch
+
+    def spy():
+        seen.append(len(srv.busy))
+        orig()
+
+    srv._launch = spy
+    rep = srv.serve(poisson_requests(12, rate=5000.0, n_lo=10, n_hi=40), realtime=False)
+    assert rep["requests"] == 12 and max(seen) <= 3
+    with pytest.raises(MemoryError):
+        RelaxServer(opts, n_slots=4, n_max=40, hbm_budget_bytes=per - 1)
