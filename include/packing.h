@@ -152,9 +152,17 @@ typedef struct {
     int32_t* status_out;       /* [n_slots] or NULL                                   */
     float* loss_out;           /* [n_slots] or NULL                                   */
     unsigned long long* stats; /* [2] tree pairs tested-in, pairs clipped/looked up   */
+    int hist_lds;              /* L-BFGS history resident in LDS during the call:
+                                  -1 when it fits, 0 never, 1 always (PK_ERANGE if
+                                  it does not fit)                                    */
 } pk_relax_params;
 
 int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
+
+/* Dynamic LDS bytes per workgroup of pk_relax_step for the given mode, largest tree count,
+ * history length and block (64, 128, 256), with the history in LDS (hist_lds 1) or not (0). */
+int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max, int history, int hist_lds,
+                       int block);
 
 /*
  * Batched L-BFGS for a caller-evaluated objective: the drop-in for
@@ -220,8 +228,9 @@ int pk_lap_hungarian(int dtype, const void* cost, int B, int N, int32_t* row_mat
  */
 int pk_diversity(const float* pop, const float* ref, int n_pop, int n_ref, int n_trees, float cos_a,
                  float sin_a, int mirror, float* out, void* stream);
+
+/* sizeof the per-slot L-BFGS record of pk_relax_step (PK_LBFGS_SLOT_BYTES). */
 int pk_relax_slot_bytes(void);
-int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,7 @@ _SIGS = {
     "pk_lut_overlap": (i32, [vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp]),
     "pk_relax_step": (i32, [vp, vp, vp]),
     "pk_relax_slot_bytes": (i32, []),
-    "pk_relax_lds_bytes": (i32, [vp, i32, i32]),
+    "pk_relax_lds_bytes": (i32, [vp, i32, i32, i32, i32, i32]),
     "pk_feasible": (i32, [vp, i32, vp, vp, i32, i32, vp, vp]),
     "pk_lbfgs_sys_bytes": (i32, []),
     "pk_lbfgs_begin": (i32, [vp, vp]),
@@ -63,6 +63,7 @@ class RelaxParams(C.Structure):
         ("lr", C.c_float), ("max_step", C.c_float), ("tol_grad", C.c_float),
         ("tol_change", C.c_float), ("tol_rel", C.c_float), ("stop_on_increase", i32),
         ("iters", i32), ("block", i32), ("status_out", vp), ("loss_out", vp), ("stats", vp),
+        ("hist_lds", i32),
     ]
 
 

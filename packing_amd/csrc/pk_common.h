@@ -75,10 +75,11 @@ struct Acc;
 template <>
 struct Acc<float> {
     using type = unsigned long long;
-    __device__ static inline void add(type* p, float v) {
-        long long q = __double2ll_rn((double)v * FIX_SCALE);
-        atomicAdd(p, (unsigned long long)q);
+    __device__ static inline type local(float v) {
+        return (unsigned long long)__double2ll_rn((double)v * FIX_SCALE);
     }
+    __device__ static inline void add(type* p, float v) { atomicAdd(p, local(v)); }
+    __device__ static inline void add_raw(type* p, type v) { atomicAdd(p, v); }
     __device__ static inline float get(type v) { return (float)((double)(long long)v * FIX_INV); }
     __device__ static inline type zero() { return 0ull; }
 };
@@ -86,7 +87,9 @@ struct Acc<float> {
 template <>
 struct Acc<double> {
     using type = double;
+    __device__ static inline type local(double v) { return v; }
     __device__ static inline void add(type* p, double v) { atomicAdd(p, v); }
+    __device__ static inline void add_raw(type* p, type v) { atomicAdd(p, v); }
     __device__ static inline double get(type v) { return v; }
     __device__ static inline type zero() { return 0.0; }
 };
@@ -104,6 +107,42 @@ __host__ __device__ inline double fast_rcp(double x) { return 1.0 / x; }
 
 __device__ inline void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
 __device__ inline void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
+
+// Orders LDS traffic between the lanes of one wave (no workgroup barrier).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave-wide reductions in a fixed order (DPP within rows of 16 lanes, then the four row
+// results combined as (r0 + r1) + (r2 + r3)); every lane returns the same value.  The
+// full wave must be active.
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double lane_d(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double wave_sum(double v) {
+    v += dpp_d<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);   // row_half_mirror
+    v += dpp_d<0x140>(v);   // row_mirror
+    return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
+}
+__device__ inline double wave_max(double v) {
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x141>(v));
+    v = fmax(v, dpp_d<0x140>(v));
+    return fmax(fmax(lane_d(v, 0), lane_d(v, 16)), fmax(lane_d(v, 32), lane_d(v, 48)));
+}
 
 __device__ inline uint64_t lanemask_lt() {
     int lane = __lane_id();

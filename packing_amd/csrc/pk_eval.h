@@ -1,13 +1,17 @@
 // Per-instance evaluation of the exact overlap / separation cost and its gradient.
 //
 // One workgroup evaluates one packing instance.  Stages (all in LDS):
-//   1. poses -> sincos -> piece vertices rotated about their tree centre;
-//   2. tree-pair candidates i<j (round-robin order, uniform work per thread) tested
-//      against the bounding-circle cutoff (2R)^2 and compacted with wave ballots;
-//   3. piece pairs of surviving tree pairs tested against piece bounding circles and
-//      compacted the same way;
+//   1. poses -> sincos -> piece vertices rotated about their tree centre, and a hashed
+//      cell grid (cell side >= the pair cutoff);
+//   2. per wave, candidate tree pairs from each tree's own and forward-neighbour cells,
+//      tested against the bounding-circle cutoff (2R)^2 and compacted with ballots into
+//      a wave-local queue;
+//   3. the queue drains 64 entries at a time: table lookups, or piece pairs of the tree
+//      pairs tested against piece bounding circles into a second wave-local queue;
 //   4. one lane per piece pair: exact clipped area (or SAT depth) and the gradient for
-//      BOTH trees, accumulated as int64 fixed point (order independent).
+//      BOTH trees, accumulated as int64 fixed point (order independent), so the result
+//      does not depend on the enumeration order or the workgroup size.
+// Periodic images (crystal axes) use block-wide lists over all pairs and lattice cells.
 //
 // Semantics follow core/pack_cuda.py:142-546: self pairs skipped, cutoff
 // |dp|^2 <= (2R)^2, total = sum over unordered pairs (the reference halves its ordered
@@ -79,8 +83,19 @@ __device__ inline void load_geom(GeomT<T>& G, const GeomT<T>* __restrict__ src) 
     for (int q = threadIdx.x; q < (int)(sizeof(GeomT<T>) / sizeof(int)); q += blockDim.x) d[q] = s[q];
 }
 
-constexpr int TP_CAP = 1024;   // tree-pair list capacity (entries)
-constexpr int PP_CAP = 2048;   // piece-pair list capacity (entries)
+constexpr int TP_CAP = 1024;   // block-wide tree-pair list capacity (periodic images)
+constexpr int PP_CAP = 2048;   // block-wide piece-pair list capacity (periodic images)
+constexpr int QCAP = 128;      // per-wave queue capacity (entries), tree pairs and piece pairs
+constexpr int MAX_WAVES = 4;   // workgroups of at most 256 threads
+
+// Cell grid for candidate pairs: hashed buckets of square cells whose side is at least
+// the pair cutoff distance, so every pair within the cutoff lies in the same or an
+// adjacent cell.
+__host__ __device__ inline int grid_buckets(int N) {
+    int h = 64;
+    while (h < 2 * N) h <<= 1;
+    return h;
+}
 
 // LDS view for one instance.
 template <typename T>
@@ -95,16 +110,29 @@ struct EvalLds {
     T* s;
     T* vx;         // [N*P*MAXV] piece vertices rotated about the tree centre
     T* vy;
-    uint32_t* tp;  // [TP_CAP] tree-pair entries: i | j << 12 | cell << 24
-    uint32_t* pp;  // [PP_CAP] piece-pair entries: tp index | pi << 16 | pj << 20
-    int* wcnt;     // [16] per-wave ballot counts
-    int N, P;
+    int* head;     // [H]   grid: first tree of each bucket (-1: empty)
+    int* next;     // [N]   grid: next tree in the same bucket
+    uint32_t* cell;   // [N] grid: biased cell coordinates cx | cy << 16
+    uint32_t* wq;  // [waves * 2 * QCAP] per-wave tree-pair and piece-pair queues
+    uint32_t* tp;  // [TP_CAP] periodic images: tree-pair entries i | j << 12 | cell << 24
+    uint32_t* pp;  // [PP_CAP] periodic images: piece-pair entries tp index | pi << 16 | pj << 20
+    uint32_t* nb;  // [nb_cap] neighbour list (relaxation): pairs a | b << 12 within the
+                   // cutoff plus a skin at the last rebuild; wcnt[8] holds its length
+    int* wcnt;     // [16] per-wave ballot counts; [8] neighbour-list length
+    double* wsum;  // [MAX_WAVES] per-wave cost partials (double precision)
+    int N, P, H, nb_cap;
 };
+
+// Neighbour-list capacity for N trees (pairs).
+__host__ __device__ inline int nb_capacity(int N) { return 6 * N + 64; }
 
 __host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Bytes of dynamic LDS for one instance of N trees with P pieces (P = 0: no vertices).
+// `periodic` selects the block-wide lists of the periodic-image path instead of the grid.
 template <typename T>
-__host__ __device__ inline size_t eval_lds_bytes(int N, int P) {
+__host__ __device__ inline size_t eval_lds_bytes(int N, int P, bool periodic = false, int waves = MAX_WAVES,
+                                                 bool list = false) {
     size_t b = 0;
     b += sizeof(typename Acc<T>::type) * (3 * (size_t)N + 1);
     b = align_up(b, 16);
@@ -112,13 +140,22 @@ __host__ __device__ inline size_t eval_lds_bytes(int N, int P) {
     b = align_up(b, 16);
     b += sizeof(T) * 2 * (size_t)N * P * MAXV;
     b = align_up(b, 16);
-    b += sizeof(uint32_t) * (TP_CAP + PP_CAP);
+    if (periodic) {
+        b += sizeof(uint32_t) * (TP_CAP + PP_CAP);
+    } else {
+        b += sizeof(int) * ((size_t)grid_buckets(N) + 2 * (size_t)N);
+        b = align_up(b, 16);
+        b += sizeof(uint32_t) * waves * 2 * QCAP;
+        if (list) b += sizeof(uint32_t) * (size_t)nb_capacity(N);
+    }
     b += sizeof(int) * 16;
+    b += sizeof(double) * MAX_WAVES;
     return align_up(b, 16);
 }
 
 template <typename T>
-__device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P) {
+__device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P, bool periodic = false,
+                                            int waves = MAX_WAVES, bool list = false) {
     EvalLds<T> L;
     size_t o = 0;
     L.g = reinterpret_cast<typename Acc<T>::type*>(base + o);
@@ -133,9 +170,27 @@ __device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P) {
     L.vy = L.vx + (size_t)N * P * MAXV;
     o += sizeof(T) * 2 * (size_t)N * P * MAXV;
     o = align_up(o, 16);
-    L.tp = reinterpret_cast<uint32_t*>(base + o);
-    L.pp = L.tp + TP_CAP;
-    L.wcnt = reinterpret_cast<int*>(L.pp + PP_CAP);
+    L.H = grid_buckets(N);
+    if (periodic) {
+        L.tp = reinterpret_cast<uint32_t*>(base + o);
+        L.pp = L.tp + TP_CAP;
+        L.wcnt = reinterpret_cast<int*>(L.pp + PP_CAP);
+        L.head = L.next = nullptr;
+        L.cell = L.wq = L.nb = nullptr;
+        L.nb_cap = 0;
+    } else {
+        L.head = reinterpret_cast<int*>(base + o);
+        L.next = L.head + L.H;
+        L.cell = reinterpret_cast<uint32_t*>(L.next + N);
+        o += sizeof(int) * ((size_t)L.H + 2 * (size_t)N);
+        o = align_up(o, 16);
+        L.wq = reinterpret_cast<uint32_t*>(base + o);
+        L.nb = L.wq + waves * 2 * QCAP;
+        L.nb_cap = list ? nb_capacity(N) : 0;
+        L.wcnt = reinterpret_cast<int*>(L.nb + L.nb_cap);
+        L.tp = L.pp = nullptr;
+    }
+    L.wsum = reinterpret_cast<double*>(L.wcnt + 16);
     L.N = N; L.P = P;
     return L;
 }
@@ -369,19 +424,56 @@ struct Crystal {
     int only_self;
 };
 
+// Grid cells: coordinates scaled by 1/side, clamped to +-30000 cells and biased by 2^15
+// into 16 bits each.  Stencil s = 0 is the tree's own cell, s = 1..4 the forward
+// neighbours (1,-1), (1,0), (1,1), (0,1): together each unordered pair of adjacent
+// cells is visited from exactly one side.
+template <typename T>
+__device__ inline uint32_t cell_key(T fx, T fy) {
+    fx = fmin(fmax(fx, T(-30000)), T(30000));
+    fy = fmin(fmax(fy, T(-30000)), T(30000));
+    const uint32_t cx = (uint32_t)((int)floor(fx) + 32768);
+    const uint32_t cy = (uint32_t)((int)floor(fy) + 32768);
+    return cx | (cy << 16);
+}
+__device__ inline uint32_t cell_step(uint32_t key, int s) {
+    const int dx = s == 4 ? 0 : 1;
+    const int dy = s == 1 ? -1 : (s == 2 ? 0 : 1);
+    return key + (uint32_t)dx + (uint32_t)(dy * 65536);
+}
+__device__ inline int cell_bucket(uint32_t key, int H) {
+    uint32_t h = (key & 0xffffu) * 0x9E3779B1u + (key >> 16) * 0x7FEB352Du;
+    h ^= h >> 16;
+    return (int)(h & (uint32_t)(H - 1));
+}
+// Reciprocal cell side: the side is 1.01 x the pair cutoff distance, so two trees
+// within the cutoff (coordinates and products rounded) never lie two cells apart.
+template <typename T>
+__device__ inline T grid_inv_side(const GeomT<T>& G) {
+    return T(1) / (T(1.01) * sqrt(G.cutoff_sq));
+}
+
 template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK>
 struct Evaluator {
     using AccT = Acc<T>;
+    using A = typename AccT::type;
+    static constexpr int NW = BLOCK / WAVE;
+    static_assert(NW <= MAX_WAVES, "at most 4 waves per workgroup");
     const GeomT<T>& G;
     EvalLds<T>& L;
     T cax, cay, cbx, cby;   // crystal axes
     int npd, only_self;
     LutDev lut;             // MODE_LUT only
-    long long n_tp, n_pp;   // statistics: tree pairs within cutoff, piece pairs clipped
+    // statistics, this wave's share (summed over waves by the caller): tree pairs
+    // within the cutoff, and piece pairs clipped or table lookups made
+    long long n_tp, n_pp;
+    A lane_cost;            // this lane's share of the cost, added to L.cost by run()
+    T cut_list;             // squared neighbour-list radius (list_mode 1)
 
-    __device__ Evaluator(const GeomT<T>& g, EvalLds<T>& l) : G(g), L(l), n_tp(0), n_pp(0) {}
+    __device__ Evaluator(const GeomT<T>& g, EvalLds<T>& l)
+        : G(g), L(l), n_tp(0), n_pp(0), lane_cost(AccT::zero()), cut_list(T(0)) {}
 
-    // Relative vector from tree i to the (shifted) tree j of a tree-pair entry.
+    // Relative vector from tree i to the (shifted) tree j of a periodic tree-pair entry.
     __device__ inline void rel(uint32_t e, int& i, int& j, T& dx, T& dy, bool& self) const {
         i = e & 0xfff;
         j = (e >> 12) & 0xfff;
@@ -397,6 +489,235 @@ struct Evaluator {
         }
     }
 
+    // Exact cost of piece pi of tree i against piece pj of tree j shifted by (dx, dy)
+    // relative to tree i; cost to this lane, gradient to both trees.
+    __device__ inline void piece_pair(int i, int j, int pi, int pj, T dx, T dy, bool self) {
+        T ax[MAXV], ay[MAXV], bx[MAXV], by[MAXV];
+        const int oa = (i * L.P + pi) * MAXV, ob = (j * L.P + pj) * MAXV;
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            ax[v] = L.vx[oa + v];
+            ay[v] = L.vy[oa + v];
+            bx[v] = L.vx[ob + v] + dx;
+            by[v] = L.vy[ob + v] + dy;
+        }
+        T gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0};
+        T c;
+        if (MODE == MODE_AREA) {
+            c = area_pair<T, GRAD>(ax, ay, bx, by, dx, dy, gi, gj);
+        } else {
+            const T sep = sat_pair<T, GRAD>(ax, ay, bx, by, dx, dy, gi, gj);
+            c = sep * sep;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { gi[k] *= T(2) * sep; gj[k] *= T(2) * sep; }
+        }
+        // Self images (periodic only) count twice when only they are summed and carry
+        // no gradient, as core/pack_cuda.py:349,452-462.
+        if (CRYSTAL && self && only_self) c *= T(2);
+        lane_cost += AccT::local(c);
+        if (GRAD && !self) {
+            AccT::add(&L.g[3 * i + 0], gi[0]);
+            AccT::add(&L.g[3 * i + 1], gi[1]);
+            AccT::add(&L.g[3 * i + 2], gi[2]);
+            AccT::add(&L.g[3 * j + 0], gj[0]);
+            AccT::add(&L.g[3 * j + 1], gj[1]);
+            AccT::add(&L.g[3 * j + 2], gj[2]);
+        }
+    }
+
+    // Table lookup for tree pair (i, j) (core/pack_cuda_lut.py:420-547): relative pose
+    // of j in i's frame, value (optionally max(0,v)^2) and gradient to both trees.
+    __device__ inline void lut_pair(int i, int j) {
+        const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+        const float cr = L.c[i], sr = L.s[i];
+        const float dxl = cr * dx + sr * dy, dyl = -sr * dx + cr * dy;
+        const float dth = wrap_pi(L.t[j] - L.t[i]);
+        float gl[3];
+        float v = lut_lookup(lut, dxl, dyl, dth, gl);
+        if (lut.quad) {
+            const float k = v > 0.f ? 2.f * v : 0.f;
+            gl[0] *= k; gl[1] *= k; gl[2] *= k;
+            v = v > 0.f ? v * v : 0.f;
+        }
+        lane_cost += AccT::local(v);
+        if (GRAD) {
+            AccT::add(&L.g[3 * i + 0], -gl[0] * cr + gl[1] * sr);
+            AccT::add(&L.g[3 * i + 1], -gl[0] * sr - gl[1] * cr);
+            AccT::add(&L.g[3 * i + 2], gl[0] * dyl - gl[1] * dxl - gl[2]);
+            AccT::add(&L.g[3 * j + 0], gl[0] * cr - gl[1] * sr);
+            AccT::add(&L.g[3 * j + 1], gl[0] * sr + gl[1] * cr);
+            AccT::add(&L.g[3 * j + 2], gl[2]);
+        }
+    }
+
+    // ---------------------------------------------------------------- grid path
+    // Piece-pair queue entry: i | j << 12 | pi << 24 | pj << 28.
+    __device__ inline void clip_entry(uint32_t e) {
+        const int i = e & 0xfff, j = (e >> 12) & 0xfff, pi = (e >> 24) & 0xf, pj = e >> 28;
+        piece_pair(i, j, pi, pj, L.x[j] - L.x[i], L.y[j] - L.y[i], false);
+    }
+
+    // Expand the tn tree pairs of tq into the piece pairs whose bounding circles meet,
+    // clipping 64 at a time from the wave's piece-pair queue.
+    __device__ inline void expand(const uint32_t* tq, int tn, uint32_t* pq, int& pn) {
+        const int lane = threadIdx.x % WAVE;
+        const int P = L.P, PP = P * P;
+        const float iPP = 1.f / (float)PP, iP = 1.f / (float)P;
+        const int items = tn * PP;
+        for (int u0 = 0; u0 < items; u0 += WAVE) {
+            const int u = u0 + lane;
+            bool pred = false;
+            uint32_t entry = 0;
+            if (u < items) {
+                // exact: u, PP < 2^15, and (u + 1/2) / PP is at least 1/(2 PP) from an integer
+                const int q = (int)(((float)u + 0.5f) * iPP);
+                const int r = u - q * PP;
+                const int pi = (int)(((float)r + 0.5f) * iP);
+                const int pj = r - pi * P;
+                const uint32_t e = tq[q];
+                const int i = e & 0xfff, j = (e >> 12) & 0xfff;
+                const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                const T cxa = L.c[i] * G.ccx[pi] - L.s[i] * G.ccy[pi];
+                const T cya = L.s[i] * G.ccx[pi] + L.c[i] * G.ccy[pi];
+                const T cxb = L.c[j] * G.ccx[pj] - L.s[j] * G.ccy[pj] + dx;
+                const T cyb = L.s[j] * G.ccx[pj] + L.c[j] * G.ccy[pj] + dy;
+                const T rr = G.cr[pi] + G.cr[pj];
+                const T ddx = cxb - cxa, ddy = cyb - cya;
+                pred = ddx * ddx + ddy * ddy <= rr * rr;
+                entry = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
+            }
+            const uint64_t m = __ballot(pred);
+            if (m) {
+                if (pred) pq[pn + __popcll(m & lanemask_lt())] = entry;
+                pn += __popcll(m);
+                wave_sync();
+                if (pn >= WAVE) {
+                    clip_entry(pq[pn - WAVE + lane]);
+                    pn -= WAVE;
+                    n_pp += WAVE;
+                    wave_sync();
+                }
+            }
+        }
+    }
+
+    // Ballot-append a candidate to the wave's tree-pair queue; drain 64 entries when it
+    // holds at least 64 (table lookups, or expansion into piece pairs).  Wave-uniform.
+    __device__ inline void push(bool pred, uint32_t entry, uint32_t* tq, int& tn, uint32_t* pq, int& pn) {
+        const int lane = threadIdx.x % WAVE;
+        const uint64_t m = __ballot(pred);
+        if (m) {
+            if (pred) tq[tn + __popcll(m & lanemask_lt())] = entry;
+            tn += __popcll(m);
+            n_tp += __popcll(m);
+            wave_sync();
+            if (tn >= WAVE) {
+                if (MODE == MODE_LUT) {
+                    const uint32_t e = tq[tn - WAVE + lane];
+                    lut_pair(e & 0xfff, (e >> 12) & 0xfff);
+                    tn -= WAVE;
+                    n_pp += WAVE;
+                } else {
+                    expand(tq, tn, pq, pn);
+                    tn = 0;
+                }
+                wave_sync();
+            }
+        }
+    }
+
+    // Candidate pairs wave by wave, into the wave-local queue; no workgroup barrier until
+    // the caller's.  list_mode 0: from the cell grid (each lane walks the stencil buckets
+    // of one tree); 1: the same, also storing pairs within the list radius (cut_list) as
+    // the neighbour list; 2: from the neighbour list.  Every mode passes exactly the pairs
+    // within the cutoff to the queue, so the result does not depend on the mode.
+    __device__ inline void run_grid(int list_mode) {
+        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        uint32_t* tq = L.wq + wave * 2 * QCAP;
+        uint32_t* pq = tq + QCAP;
+        int tn = 0, pn = 0;
+        const int N = L.N;
+        if (list_mode == 2) {
+            const int nl = L.wcnt[8];
+            for (int base = wave * WAVE; base < nl; base += BLOCK) {
+                const int k = base + lane;
+                bool pred = false;
+                uint32_t e = 0;
+                if (k < nl) {
+                    e = L.nb[k];
+                    const int i = e & 0xfff, j = (e >> 12) & 0xfff;
+                    const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                    pred = dx * dx + dy * dy <= G.cutoff_sq;
+                }
+                push(pred, e, tq, tn, pq, pn);
+            }
+        } else {
+            for (int base = wave * WAVE; base < N; base += BLOCK) {
+                const int i = base + lane;
+                bool alive = i < N;
+                uint32_t ci = 0, target = 0;
+                T xi = T(0), yi = T(0);
+                int s = 0, j = -1;
+                if (alive) {
+                    ci = L.cell[i];
+                    target = ci;
+                    xi = L.x[i];
+                    yi = L.y[i];
+                    j = L.head[cell_bucket(ci, L.H)];
+                }
+                while (true) {
+                    bool pred = false, pl = false;
+                    uint32_t entry = 0;
+                    if (alive) {
+                        while (j < 0 && s < 4) {
+                            ++s;
+                            target = cell_step(ci, s);
+                            j = L.head[cell_bucket(target, L.H)];
+                        }
+                        if (j < 0) {
+                            alive = false;
+                        } else {
+                            const int jj = j;
+                            j = L.next[jj];
+                            if (L.cell[jj] == target && (s > 0 || jj > i)) {
+                                const T dx = L.x[jj] - xi, dy = L.y[jj] - yi;
+                                const T d2 = dx * dx + dy * dy;
+                                pred = d2 <= G.cutoff_sq;
+                                pl = d2 <= cut_list;
+                                const int a = i < jj ? i : jj, b = i < jj ? jj : i;
+                                entry = (uint32_t)a | ((uint32_t)b << 12);
+                            }
+                        }
+                    }
+                    if (list_mode == 1) {
+                        const uint64_t ml = __ballot(pl);
+                        if (ml) {
+                            int off = 0;
+                            if (lane == 0) off = atomicAdd(&L.wcnt[8], __popcll(ml));
+                            off = __shfl(off, 0);
+                            const int q = off + __popcll(ml & lanemask_lt());
+                            if (pl && q < L.nb_cap) L.nb[q] = entry;
+                        }
+                    }
+                    push(pred, entry, tq, tn, pq, pn);
+                    if (__ballot(alive) == 0) break;
+                }
+            }
+        }
+        if (MODE == MODE_LUT) {
+            if (lane < tn) {
+                const uint32_t e = tq[lane];
+                lut_pair(e & 0xfff, (e >> 12) & 0xfff);
+            }
+            n_pp += tn;
+        } else {
+            expand(tq, tn, pq, pn);
+            if (lane < pn) clip_entry(pq[lane]);
+            n_pp += pn;
+        }
+    }
+
+    // ----------------------------------------------------- periodic-image path
     __device__ inline void flush_pp(int n) {
         for (int q = threadIdx.x; q < n; q += BLOCK) {
             const uint32_t e = L.pp[q];
@@ -406,82 +727,15 @@ struct Evaluator {
             T dx, dy;
             bool self;
             rel(te, i, j, dx, dy, self);
-            T ax[MAXV], ay[MAXV], bx[MAXV], by[MAXV];
-            const int oa = (i * L.P + pi) * MAXV, ob = (j * L.P + pj) * MAXV;
-#pragma unroll
-            for (int v = 0; v < MAXV; ++v) {
-                ax[v] = L.vx[oa + v];
-                ay[v] = L.vy[oa + v];
-                bx[v] = L.vx[ob + v] + dx;
-                by[v] = L.vy[ob + v] + dy;
-            }
-            T gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0};
-            T c;
-            if (MODE == MODE_AREA) {
-                c = area_pair<T, GRAD>(ax, ay, bx, by, dx, dy, gi, gj);
-            } else {
-                const T sep = sat_pair<T, GRAD>(ax, ay, bx, by, dx, dy, gi, gj);
-                c = sep * sep;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) { gi[k] *= T(2) * sep; gj[k] *= T(2) * sep; }
-            }
-            // Self images (periodic only) count twice when only they are summed and carry
-            // no gradient, as core/pack_cuda.py:349,452-462.
-            if (CRYSTAL && self && only_self) c *= T(2);
-            AccT::add(L.cost, c);
-            if (GRAD && !self) {
-                AccT::add(&L.g[3 * i + 0], gi[0]);
-                AccT::add(&L.g[3 * i + 1], gi[1]);
-                AccT::add(&L.g[3 * i + 2], gi[2]);
-                AccT::add(&L.g[3 * j + 0], gj[0]);
-                AccT::add(&L.g[3 * j + 1], gj[1]);
-                AccT::add(&L.g[3 * j + 2], gj[2]);
-            }
+            piece_pair(i, j, pi, pj, dx, dy, self);
         }
-        n_pp += n;
-        __syncthreads();
-    }
-
-    // Table lookups for the tree pairs of the list (core/pack_cuda_lut.py:420-547):
-    // relative pose of j in i's frame, value (optionally max(0,v)^2) and gradient to
-    // both trees.
-    __device__ inline void lut_tp(int ntp) {
-        for (int q = threadIdx.x; q < ntp; q += BLOCK) {
-            int i, j;
-            T dx, dy;
-            bool self;
-            rel(L.tp[q], i, j, dx, dy, self);
-            const float cr = L.c[i], sr = L.s[i];
-            const float dxl = cr * dx + sr * dy, dyl = -sr * dx + cr * dy;
-            const float dth = wrap_pi(L.t[j] - L.t[i]);
-            float gl[3];
-            float v = lut_lookup(lut, dxl, dyl, dth, gl);
-            if (lut.quad) {
-                const float k = v > 0.f ? 2.f * v : 0.f;
-                gl[0] *= k; gl[1] *= k; gl[2] *= k;
-                v = v > 0.f ? v * v : 0.f;
-            }
-            AccT::add(L.cost, v);
-            if (GRAD) {
-                AccT::add(&L.g[3 * i + 0], -gl[0] * cr + gl[1] * sr);
-                AccT::add(&L.g[3 * i + 1], -gl[0] * sr - gl[1] * cr);
-                AccT::add(&L.g[3 * i + 2], gl[0] * dyl - gl[1] * dxl - gl[2]);
-                AccT::add(&L.g[3 * j + 0], gl[0] * cr - gl[1] * sr);
-                AccT::add(&L.g[3 * j + 1], gl[0] * sr + gl[1] * cr);
-                AccT::add(&L.g[3 * j + 2], gl[2]);
-            }
-        }
-        n_pp += ntp;
+        if (threadIdx.x < WAVE) n_pp += n;
         __syncthreads();
     }
 
     // Expand the tree-pair list into piece pairs (circle test), clipping as the
     // piece-pair list fills.
     __device__ inline void flush_tp(int ntp) {
-        if (MODE == MODE_LUT) {
-            lut_tp(ntp);
-            return;
-        }
         const int PP = L.P * L.P;
         const int nitems = ntp * PP;
         int npp = 0;
@@ -519,92 +773,80 @@ struct Evaluator {
         flush_pp(npp);
     }
 
-    // Enumerate candidate tree pairs and process them.
-    __device__ inline void run() {
+    // Pairs i <= j times lattice cells; a tree's own images only for the positive half
+    // of the cells (image s and -s have equal overlap).
+    __device__ inline void run_periodic() {
         const int N = L.N;
         int ntp = 0;
-        if (!CRYSTAL) {
-            // Round-robin order: pairs (i, (i + d) mod N), d = 1..D, then d = N/2 for
-            // i < N/2 when N is even; every unordered pair exactly once.
-            const int D = (N - 1) / 2;
-            const long long nfull = (long long)N * D;
-            const long long ncand = (long long)N * (N - 1) / 2;
-            for (long long base = 0; base < ncand; base += BLOCK) {
-                const long long k = base + threadIdx.x;
-                bool pred = false;
-                uint32_t entry = 0;
-                if (k < ncand) {
-                    int i, d;
-                    if (k < nfull) {
-                        d = (int)(k / N) + 1;
-                        i = (int)(k - (long long)(d - 1) * N);
-                    } else {
-                        i = (int)(k - nfull);
-                        d = N / 2;
-                    }
-                    int j = i + d;
-                    j = j >= N ? j - N : j;
-                    const int a = i < j ? i : j, b = i < j ? j : i;
-                    const T dx = L.x[b] - L.x[a], dy = L.y[b] - L.y[a];
-                    pred = dx * dx + dy * dy <= G.cutoff_sq;
-                    entry = (uint32_t)a | ((uint32_t)b << 12);
-                }
-                int total;
-                const int slot = block_append<BLOCK>(pred, L.wcnt, total);
-                if (ntp + total > TP_CAP) {
-                    flush_tp(ntp);
-                    ntp = 0;
-                }
-                if (pred) L.tp[ntp + slot] = entry;
-                ntp += total;
-                n_tp += total;
-                __syncthreads();
+        const int nc = 2 * npd + 1, ncell = nc * nc, center = (ncell - 1) / 2;
+        const long long npair = (long long)N * (N + 1) / 2;
+        const long long ncand = npair * ncell;
+        for (long long base = 0; base < ncand; base += BLOCK) {
+            const long long k = base + threadIdx.x;
+            bool pred = false;
+            uint32_t entry = 0;
+            if (k < ncand) {
+                const int cell = (int)(k % ncell);
+                long long q = k / ncell;
+                int i = 0;
+                while (q >= N - i) { q -= N - i; ++i; }
+                const int j = i + (int)q;
+                const bool self = i == j;
+                bool ok = self ? cell > center : !only_self;
+                const uint32_t e = (uint32_t)i | ((uint32_t)j << 12) | ((uint32_t)cell << 24);
+                int ii, jj;
+                T dx, dy;
+                bool sf;
+                rel(e, ii, jj, dx, dy, sf);
+                pred = ok && (dx * dx + dy * dy <= G.cutoff_sq);
+                entry = e;
             }
-        } else {
-            // Pairs i <= j times lattice cells; a tree's own images only for the positive
-            // half of the cells (image s and -s have equal overlap).
-            const int nc = 2 * npd + 1, ncell = nc * nc, center = (ncell - 1) / 2;
-            const long long npair = (long long)N * (N + 1) / 2;
-            const long long ncand = npair * ncell;
-            for (long long base = 0; base < ncand; base += BLOCK) {
-                const long long k = base + threadIdx.x;
-                bool pred = false;
-                uint32_t entry = 0;
-                if (k < ncand) {
-                    const int cell = (int)(k % ncell);
-                    long long q = k / ncell;
-                    int i = 0;
-                    while (q >= N - i) { q -= N - i; ++i; }
-                    const int j = i + (int)q;
-                    const bool self = i == j;
-                    bool ok = self ? cell > center : !only_self;
-                    const uint32_t e = (uint32_t)i | ((uint32_t)j << 12) | ((uint32_t)cell << 24);
-                    int ii, jj;
-                    T dx, dy;
-                    bool sf;
-                    rel(e, ii, jj, dx, dy, sf);
-                    pred = ok && (dx * dx + dy * dy <= G.cutoff_sq);
-                    entry = e;
-                }
-                int total;
-                const int slot = block_append<BLOCK>(pred, L.wcnt, total);
-                if (ntp + total > TP_CAP) {
-                    flush_tp(ntp);
-                    ntp = 0;
-                }
-                if (pred) L.tp[ntp + slot] = entry;
-                ntp += total;
-                n_tp += total;
-                __syncthreads();
+            int total;
+            const int slot = block_append<BLOCK>(pred, L.wcnt, total);
+            if (ntp + total > TP_CAP) {
+                flush_tp(ntp);
+                ntp = 0;
             }
+            if (pred) L.tp[ntp + slot] = entry;
+            ntp += total;
+            if (threadIdx.x < WAVE) n_tp += total;
+            __syncthreads();
         }
         flush_tp(ntp);
     }
+
+    // Evaluate every candidate pair; the caller synchronises the workgroup afterwards and
+    // reads the cost with cost_total().  list_mode as run_grid (neighbour list: relaxation
+    // only).  Float costs are summed in fixed point; double costs per wave in a fixed lane
+    // order, then over the waves in order, so both are reproducible.
+    __device__ inline void run(int list_mode = 0) {
+        if (CRYSTAL)
+            run_periodic();
+        else
+            run_grid(list_mode);
+        if constexpr (sizeof(T) == 4) {
+            if (lane_cost != AccT::zero()) AccT::add_raw(L.cost, lane_cost);
+        } else {
+            const double w = wave_sum(lane_cost);
+            if (threadIdx.x % WAVE == 0) L.wsum[threadIdx.x / WAVE] = w;
+        }
+    }
+
+    // Total cost, after the workgroup barrier that follows run().
+    __device__ inline T cost_total() const {
+        if constexpr (sizeof(T) == 4) {
+            return AccT::get(*L.cost);
+        } else {
+            double c = 0.0;
+            for (int w = 0; w < NW; ++w) c += L.wsum[w];
+            return c;
+        }
+    }
 };
 
-// Stage poses (already in L.x/y/t) into sincos and rotated piece vertices; clear the
-// accumulators.  Ends with a barrier.
-template <typename T, int BLOCK, bool VERTS = true>
+// Stage poses (already in L.x/y/t) into sincos, rotated piece vertices (VERTS) and the
+// cell grid (GRID); clear the accumulators.  Ends with a barrier.
+template <typename T, int BLOCK, bool VERTS = true, bool GRID = true>
 __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
     const int N = L.N, P = L.P;
     for (int i = threadIdx.x; i < N; i += BLOCK) {
@@ -614,17 +856,28 @@ __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
         L.s[i] = sn;
     }
     for (int q = threadIdx.x; q < 3 * N + 1; q += BLOCK) L.g[q] = Acc<T>::zero();
+    if (GRID)
+        for (int h = threadIdx.x; h < L.H; h += BLOCK) L.head[h] = -1;
     __syncthreads();
-    if (!VERTS) return;
-    const int nv = N * P * MAXV;
-    for (int q = threadIdx.x; q < nv; q += BLOCK) {
-        const int i = q / (P * MAXV);
-        const int r = q - i * P * MAXV;
-        const T lx = G.px[r], ly = G.py[r];
-        L.vx[q] = L.c[i] * lx - L.s[i] * ly;
-        L.vy[q] = L.s[i] * lx + L.c[i] * ly;
+    if (VERTS) {
+        const int nv = N * P * MAXV;
+        for (int q = threadIdx.x; q < nv; q += BLOCK) {
+            const int i = q / (P * MAXV);
+            const int r = q - i * P * MAXV;
+            const T lx = G.px[r], ly = G.py[r];
+            L.vx[q] = L.c[i] * lx - L.s[i] * ly;
+            L.vy[q] = L.s[i] * lx + L.c[i] * ly;
+        }
     }
-    __syncthreads();
+    if (GRID) {
+        const T inv = grid_inv_side(G);
+        for (int i = threadIdx.x; i < N; i += BLOCK) {
+            const uint32_t key = cell_key(L.x[i] * inv, L.y[i] * inv);
+            L.cell[i] = key;
+            L.next[i] = atomicExch(&L.head[cell_bucket(key, L.H)], i);
+        }
+    }
+    if (VERTS || GRID) __syncthreads();
 }
 
 // Boundary term of one tree (core/pack_cuda.py:550-642): max over outline vertices of
@@ -633,15 +886,36 @@ __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
 template <typename T>
 __device__ inline T boundary_tree(const GeomT<T>& G, T x, T y, T c, T s, T half, T hx, T hy,
                                   T (&g)[3], T (&gh)[3]) {
+    // A tree whose bounding circle (radius plus a margin far above the rounding of the
+    // vertex positions below) lies inside the square has no vertex outside: cost and
+    // gradients are zero, as the vertex loop would find.
+    const T rx = fabs(x - hx), ry = fabs(y - hy);
+    const T margin = G.radius * T(1e-4) + T(1e-5) * (T(1) + fabs(x) + fabs(y) + fabs(hx) + fabs(hy) + half);
+    if (rx + G.radius + margin <= half && ry + G.radius + margin <= half) {
+        g[0] = g[1] = g[2] = T(0);
+        gh[0] = gh[1] = gh[2] = T(0);
+        return T(0);
+    }
     T best = T(0);
     int bi = 0;
-    for (int v = 0; v < G.n_outline; ++v) {
-        const T lx = G.ox[v], ly = G.oy[v];
-        const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
-        const T ax = fabs(vx), ay = fabs(vy);
-        const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);
-        const T d2 = dx * dx + dy * dy;
-        if (d2 > best) { best = d2; bi = v; }
+    const int no = G.n_outline;
+    for (int v0 = 0; v0 < no; v0 += 8) {   // blocks of 8 vertices, loads issued together
+        T lxs[8], lys[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int v = min(v0 + u, no - 1);
+            lxs[u] = G.ox[v];
+            lys[u] = G.oy[v];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const T lx = lxs[u], ly = lys[u];
+            const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
+            const T ax = fabs(vx), ay = fabs(vy);
+            const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);
+            const T d2 = dx * dx + dy * dy;
+            if (v0 + u < no && d2 > best) { best = d2; bi = v0 + u; }
+        }
     }
     const T lx = G.ox[bi], ly = G.oy[bi];
     const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void k_lut_overlap(const float* __restrict__
     }
     load_geom(G, geom);
     __syncthreads();
-    EvalLds<float> L = carve_eval_lds<float>(smem, N, 0);
+    EvalLds<float> L = carve_eval_lds<float>(smem, N, 0, false, BLOCK / WAVE);
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         L.x[i] = xyt[3 * (off + i) + 0];
         L.y[i] = xyt[3 * (off + i) + 1];
@@ -97,12 +97,10 @@ __global__ __launch_bounds__(BLOCK) void k_lut_overlap(const float* __restrict__
     ev.lut = lut;
     ev.run();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        out_cost[e] = Acc<float>::get(*L.cost);
-        if (stats) {
-            atomicAdd(&stats[0], (unsigned long long)ev.n_tp);
-            atomicAdd(&stats[1], (unsigned long long)ev.n_pp);
-        }
+    if (threadIdx.x == 0) out_cost[e] = ev.cost_total();
+    if (stats && threadIdx.x % WAVE == 0) {
+        atomicAdd(&stats[0], (unsigned long long)ev.n_tp);
+        atomicAdd(&stats[1], (unsigned long long)ev.n_pp);
     }
     if (out_grads)
         for (int q = threadIdx.x; q < 3 * N; q += BLOCK) out_grads[3 * off + q] = Acc<float>::get(L.g[q]);
@@ -168,7 +166,7 @@ int pk_lut_overlap(pk_geometry_t h, pk_lut_t lut, const float* xyt, const int32_
     if (n_ens == 0) return PK_OK;
     PK_REQUIRE(xyt && out_cost, "pk_lut_overlap: xyt and out_cost are required");
     const LutDev d = reinterpret_cast<const Lut*>(lut)->d;
-    const size_t lds = eval_lds_bytes<float>(n_trees, 0);
+    const size_t lds = eval_lds_bytes<float>(n_trees, 0, false, MAX_WAVES);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int blk = block ? block : 256;
 #define PK_LO(B)                                                                                   \

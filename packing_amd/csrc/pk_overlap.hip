@@ -40,14 +40,14 @@ __global__ __launch_bounds__(BLOCK) void k_overlap(const T* __restrict__ xyt,
     }
     load_geom(G, geom);
     __syncthreads();
-    EvalLds<T> L = carve_eval_lds<T>(smem, N, G.P);
+    EvalLds<T> L = carve_eval_lds<T>(smem, N, G.P, CRYSTAL, BLOCK / WAVE);
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         L.x[i] = xyt[3 * (off + i) + 0];
         L.y[i] = xyt[3 * (off + i) + 1];
         L.t[i] = xyt[3 * (off + i) + 2];
     }
     __syncthreads();
-    stage_instance<T, BLOCK>(G, L);
+    stage_instance<T, BLOCK, true, !CRYSTAL>(G, L);
     Evaluator<T, MODE, GRAD, CRYSTAL, BLOCK> ev(G, L);
     if (CRYSTAL) {
         ev.cax = crystal[4 * e + 0];
@@ -63,12 +63,10 @@ __global__ __launch_bounds__(BLOCK) void k_overlap(const T* __restrict__ xyt,
     }
     ev.run();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        out_cost[e] = Acc<T>::get(*L.cost);
-        if (stats) {
-            atomicAdd(&stats[0], (unsigned long long)ev.n_tp);
-            atomicAdd(&stats[1], (unsigned long long)ev.n_pp);
-        }
+    if (threadIdx.x == 0) out_cost[e] = ev.cost_total();
+    if (stats && threadIdx.x % WAVE == 0) {
+        atomicAdd(&stats[0], (unsigned long long)ev.n_tp);
+        atomicAdd(&stats[1], (unsigned long long)ev.n_pp);
     }
     if (GRAD)
         for (int q = threadIdx.x; q < 3 * N; q += BLOCK) out_grads[3 * off + q] = Acc<T>::get(L.g[q]);
@@ -166,7 +164,7 @@ static int run_overlap(const Geometry* g, const void* xyt, const int* inst_off, 
                        void* grads, unsigned long long* stats, hipStream_t st) {
     auto kern = k_overlap<T, MODE, GRAD, CRYSTAL, BLOCK>;
     const int P = (sizeof(T) == 4) ? g->hf.P : g->hd.P;
-    const size_t lds = eval_lds_bytes<T>(n_trees, P);
+    const size_t lds = eval_lds_bytes<T>(n_trees, P, CRYSTAL, BLOCK / WAVE);
     int rc = launch_lds(kern, n_ens, BLOCK, lds, st, sizeof(GeomT<T>));
     if (rc) return rc;
     const GeomT<T>* dg = reinterpret_cast<const GeomT<T>*>(sizeof(T) == 4 ? (const void*)g->df

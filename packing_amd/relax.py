@@ -44,7 +44,9 @@ class RelaxOptions:
     tol_change: float = 0.0
     tol_rel: float = 0.0
     stop_on_increase: bool = False
-    block: int = 0          # 0: chosen by the tile table (autotune.py)
+    block: int = 0          # 0: chosen by the tile table (autotune.py), per bucket
+    buckets: int = 4        # size buckets launched concurrently (1: one launch)
+    hist_lds: int = -1      # L-BFGS history in LDS: -1 when it fits, 0 never, 1 always
     lut: object = None      # pack_cuda_lut.LookupTable for mode "lut"
 
 
@@ -100,6 +102,22 @@ class RelaxBatch:
         self._dg = device_geometry(dev, self.geometry)
         self._graph = None
         self.x0 = self.x.clone()
+        self._buckets = self._make_buckets(order)
+        self._side = [torch.cuda.Stream(device=dev) for _ in self._buckets[1:]] if self.M else []
+
+    def _make_buckets(self, order):
+        """Split the size-sorted order into up to opts.buckets equal-count slices:
+        (offset, count, n_tree_max).  Dynamic LDS is sized per launch by its largest
+        instance, so a slice of small instances gets small workgroups and high occupancy."""
+        k = max(1, min(int(self.opts.buckets or 1), self.M))
+        if self.M == 0:
+            return []
+        per = -(-self.M // k)
+        out = []
+        for b in range(0, self.M, per):
+            cnt = min(per, self.M - b)
+            out.append((b, cnt, int(self.sizes[order[b]])))
+        return out
 
     def reset(self) -> None:
         """Back to the initial poses with fresh optimizer state (device-side copies, no
@@ -143,16 +161,46 @@ class RelaxBatch:
         p.status_out = self.status.data_ptr()
         p.loss_out = self.loss.data_ptr()
         p.stats = self.stats.data_ptr()
+        p.hist_lds = int(o.hist_lds)
         return p
 
     def _auto_block(self) -> int:
         from .autotune import choose_block
         return choose_block(self.opts.mode, self.n_tree_max, self.geometry.n_pieces)
 
-    def launch(self, iters: int, stream=None, **kw) -> None:
+    def _launch_one(self, iters, stream, **kw) -> None:
         p = self._params(iters, **kw)
         N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), N.stream_handle(stream)),
                 "pk_relax_step")
+
+    def launch(self, iters: int, stream=None, **kw) -> None:
+        """Advance by up to ``iters`` iterations.  Without explicit order/n_launch, one
+        launch per size bucket: the largest on ``stream``, the others on side streams
+        forked from and joined back to it (a graph captures the fork and join)."""
+        if kw or len(self._buckets) <= 1:
+            self._launch_one(iters, stream, **kw)
+            return
+        main = stream if stream is not None else torch.cuda.current_stream(self.device)
+        fork = torch.cuda.Event()
+        fork.record(main)
+        joins = []
+        for k, (off, cnt, nmax) in enumerate(self._buckets):
+            s = main if k == 0 else self._side[k - 1]
+            if k:
+                s.wait_event(fork)
+            blk = int(self.opts.block) or self._block_for(nmax)
+            self._launch_one(iters, s, order=self.order[off:off + cnt], n_launch=cnt,
+                             n_tree_max=nmax, block=blk)
+            if k:
+                e = torch.cuda.Event()
+                e.record(s)
+                joins.append(e)
+        for e in joins:
+            main.wait_event(e)
+
+    def _block_for(self, n_max: int) -> int:
+        from .autotune import choose_block
+        return choose_block(self.opts.mode, n_max, self.geometry.n_pieces)
 
     # ------------------------------------------------------------------ paths
     def run(self, iters_per_launch: int | None = None, stream=None) -> None:
