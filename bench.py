@@ -35,6 +35,94 @@ def make_workload(n_inst, n_lo, n_hi, seed):
     return sizes, xs, hs
 
 
+# Arithmetic of the exact area kernel (add, sub, mul, reciprocal; a fused multiply-add
+# counts 2), counted from the fixed, fully unrolled code of one evaluation:
+#   per tree:            sincos (counted 20) + rotation of P*4 piece vertices (6 each)
+#   per tree-pair test:  centre distance (5)
+#   per piece-pair test: both piece bounding-circle centres and the distance (21)
+#   per clipped piece pair: 623 (clip of A's edges against B (200, 88 of them FP64),
+#       Green's sums and shape derivative over A's edges (124), clip of B against A
+#       (152, 88 FP64), sums over B's edges (124), closing terms (8), relative vertices (8),
+#       fixed-point conversion of cost and 6 gradient terms (7))
+FLOP_TREE_BASE, FLOP_VERT, FLOP_TP, FLOP_PP_TEST, FLOP_PP = 20, 24, 5, 21, 623
+
+
+def area_eval_flops(n_trees, n_pieces, tree_pairs, piece_pairs):
+    """Algorithmic FLOP of one exact area + gradient evaluation over a batch."""
+    return (n_trees * (FLOP_TREE_BASE + FLOP_VERT * n_pieces) + tree_pairs * FLOP_TP
+            + tree_pairs * n_pieces * n_pieces * FLOP_PP_TEST + piece_pairs * FLOP_PP)
+
+
+def overlap_kernel_roofline(geom, dev, n_ens=1024, n_trees=100, reps=10, seed=1):
+    """Exact overlap area + gradient (pk_overlap) on the plain-method shape: n_ens instances
+    of n_trees trees at the benchmark's initial pose distribution.  Kernel time from HIP
+    events on the launch stream; FLOP from the kernel's own pair counters."""
+    import torch
+    from packing_amd import pack_cuda
+    from packing_amd.geometry import make_poses
+    x = make_poses(np.random.default_rng(seed), n_ens, n_trees, np.float32)
+    xt = torch.tensor(x, device=dev)
+    c = torch.empty(n_ens, device=dev)
+    g = torch.empty_like(xt)
+    st = torch.zeros(2, dtype=torch.int64, device=dev)
+    s = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(s):
+        pack_cuda.overlap_multi_ensemble(xt, xt, False, out_cost=c, out_grads=g, stream=s)
+        s.synchronize()
+        pack_cuda.overlap_multi_ensemble(xt, xt, False, out_cost=c, out_grads=g, stream=s, stats=st)
+        s.synchronize()
+        tp, pp = (int(v) for v in st.cpu().numpy())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            pack_cuda.overlap_multi_ensemble(xt, xt, False, out_cost=c, out_grads=g, stream=s)
+        e1.record(s)
+        s.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    fl = area_eval_flops(n_ens * n_trees, geom.n_pieces, tp, pp)
+    tf = fl / (ms * 1e-3) / 1e12
+    return {"kernel": "k_overlap<float, area, grad>", "bound": "valu",
+            "workload": f"{n_ens} instances x {n_trees} trees, initial poses, random 15-vertex "
+                        f"polygon ({geom.n_pieces} pieces)",
+            "avg_launch_ms": round(ms, 4), "tree_pairs": tp, "piece_pairs": pp,
+            "flop_per_launch": fl, "achieved": round(tf, 3), "unit": "TFLOP/s",
+            "peak": 157.3, "frac": round(tf / 157.3, 4),
+            "frac_of_nonpacked_fp32": round(tf / 78.6, 4)}
+
+
+def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0):
+    """Continuous-batching server under Poisson arrivals (rate per second, N log-uniform
+    in [10, 200], iters L-BFGS iterations per request) and the plain per-request path on
+    the first n_plain requests of the same stream."""
+    from packing_amd.relax import RelaxOptions
+    from packing_amd.serving import RelaxServer, poisson_requests, serve_plain
+    opts = RelaxOptions(mode="lut", lut=lut, max_iter=iters, history=8, max_step=0.1)
+    srv = RelaxServer(opts, n_slots=4096, n_max=200, hbm_budget_bytes=64e9, chunk=100, device=dev,
+                      geometry=geom)
+    warm = poisson_requests(8, rate=rate, seed_arrivals=1, seed_requests=2)
+    srv.serve(warm, realtime=False)
+    reqs = poisson_requests(n_req, rate=rate)
+    rep = srv.serve(reqs)
+    out = {"requests": rep["requests"], "rate_per_s": rate, "iters": iters,
+           "throughput": round(rep["throughput"], 2), "p50_ms": round(rep["p50_s"] * 1e3, 3),
+           "p99_ms": round(rep["p99_s"] * 1e3, 3), "launches": rep["launches"],
+           "feasible_fraction_fp64": rep["feasible_fraction"], "slots": rep["slots"],
+           "hbm_budget_gb": 64}
+    if n_plain > 0:
+        pr = poisson_requests(n_req, rate=rate)[:n_plain]
+        pl = serve_plain(pr, opts, geometry=geom, device=dev)
+        svc = np.array([r.t_done - r.t_admit for r in pr])
+        out["plain"] = {"requests": n_plain, "throughput": round(pl["throughput"], 3),
+                        "p50_ms": round(pl["p50_s"] * 1e3, 3), "p99_ms": round(pl["p99_s"] * 1e3, 3),
+                        "service_p50_ms": round(float(np.percentile(svc, 50)) * 1e3, 3),
+                        "feasible_fraction_fp64": pl["feasible_fraction"]}
+        same = all(np.array_equal(a.result.cpu().numpy(), b.result.cpu().numpy())
+                   for a, b in zip(pr, reqs[:n_plain]))
+        out["plain"]["bitwise_equal_to_served"] = bool(same)
+        out["throughput_vs_plain"] = round(out["throughput"] / max(pl["throughput"], 1e-12), 2)
+    return out
+
+
 def cpu_baseline(geom, lut, sizes, xs, hs, opts, budget_s):
     """Same workload on the host: the FP64 oracle (per-instance L-BFGS, table overlap +
     boundary), OpenMP over instances on every host core, on a bounded sample."""
@@ -74,6 +162,11 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=6)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the served session (0: skip serving)")
+    ap.add_argument("--serve-plain", type=int, default=10, help="requests timed on the plain path")
+    ap.add_argument("--serve-iters", type=int, default=2000)
+    ap.add_argument("--no-overlap", action="store_true", help="skip the exact-kernel roofline")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
@@ -170,19 +263,24 @@ def main():
     launch_ms = [a.elapsed_time(c) for a, c in evs]
     lookups = int(b.stats[1].item())
     n_state = 3 * sizes.astype(np.int64) + 3
-    it_total = int(rec[:, 0].sum())
-    # algorithmic HBM bytes: table gathers (8 corners x 4 B per pair looked up), L-BFGS
-    # history (two-loop reads s and y twice over up to 8 slots, writes one pair) and the
-    # state read/written once per launch
-    hl = np.minimum(np.maximum(rec[:, 0] - 1, 0), 8)
-    hist_bytes = float(np.sum(rec[:, 0] * (4 * hl * n_state * 4 + 2 * n_state * 4)))
-    state_bytes = float(np.sum(n_state) * 4 * 4 * n_chunks)
-    alg_bytes = lookups * 32 + hist_bytes + state_bytes
+    # Algorithmic memory bytes per chunk launch: table gathers (8 corners x 4 B per pair
+    # looked up) plus, for every instance still running at the chunk's start, its state x
+    # and gradient read and written once (16 B per state element) and its L-BFGS history
+    # (history x (s, y) rows) copied into LDS and back once (2 x 2 x history x 4 B per
+    # element); within a launch the history, gradient and poses stay in LDS.
+    it_i = rec[:, 0].astype(np.int64)
+    run_bytes = 0.0
+    for c in range(n_chunks):
+        act = (it_i > c * chunk) | (c == 0)
+        hist_rows = 0 if c == 0 else 2 * 2 * opts.history
+        run_bytes += float(np.sum(n_state[act]) * 4 * (4 + hist_rows))
+    alg_bytes = lookups * 32 + run_bytes
     avg_launch_s = float(np.mean(launch_ms)) / 1e3
     achieved = alg_bytes / n_chunks / avg_launch_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 5), "traffic": None,
-                "kernel": "k_relax<MODE_LUT>", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+                "kernel": "k_relax<MODE_LUT> (one chunk: the size-bucket launches of 100 iterations)",
+                "avg_launch_ms": round(avg_launch_s * 1e3, 3),
                 "alg_bytes_per_launch": alg_bytes / n_chunks, "lookups_per_launch": lookups / n_chunks}
 
     # plain path: one launch per instance per iteration, same kernel and numerics
@@ -204,6 +302,16 @@ def main():
                    for k, i in enumerate(pick))
         plain["bitwise_equal_to_batched"] = bool(same)
         log(f"plain: {plain}")
+
+    ovl = None
+    if not args.no_overlap:
+        ovl = overlap_kernel_roofline(geom, dev)
+        log(f"overlap kernel: {ovl}")
+
+    serving = None
+    if args.serve_requests > 0:
+        serving = serving_section(geom, lut, dev, args.serve_requests, args.serve_plain, args.serve_iters)
+        log(f"serving: {serving}")
 
     cpu = None
     if not args.no_cpu:
@@ -232,6 +340,10 @@ def main():
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "feasible_fraction_fp64": feasible_frac,
+        "iterations_mean": round(float(rec[:, 0].mean()), 2),
+        "stopped_before_max_iter": round(float((rec[:, 0] < args.iters).mean()), 4),
+        "serving": serving,
+        "overlap_kernel": ovl,
         "final_loss_median": float(np.median(loss)),
         "table_build_s": round(t_lut, 4),
         "device": info,

@@ -1,10 +1,12 @@
-"""Tile-shape choice for the per-instance kernels (threads per workgroup) and the
-chunk length of the batched relaxation.
+"""Tile-shape choice for the relaxation kernels.
 
-``choose_block`` reads a table keyed by (mode, instance-size bucket); ``tune`` measures
-the candidates on the device for a given batch and stores the winner in that table.
-Every candidate gives bitwise the same results (all reductions are fixed-order or
-fixed-point), so the choice only affects speed.
+Candidates: 1 = one wave per instance with the state and L-BFGS history in registers
+(up to 212 trees, history <= 8), or 64 / 128 / 256 threads per instance with the vectors
+in LDS.  0 lets the library choose (1 where it applies, else 256).  ``choose_block`` reads
+a table keyed by (mode, instance-size bucket); ``tune`` measures the candidates on the
+device for a given batch and stores the winner in that table.  Every candidate gives
+bitwise the same results (all reductions are fixed-order or fixed-point), so the choice
+only affects speed.
 """
 from __future__ import annotations
 
@@ -12,7 +14,8 @@ import json
 import os
 import time
 
-BLOCKS = (64, 128, 256)
+BLOCKS = (1, 64, 128, 256)
+WAVE_MAX_TREES = 212
 BUCKETS = (32, 64, 128, 256, 512, 4096)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -28,9 +31,7 @@ def _bucket(n: int) -> int:
 
 
 def _default(mode: str, n: int, n_pieces: int) -> int:
-    if mode == "lut":
-        return 64 if n <= 64 else (128 if n <= 160 else 256)
-    return 64 if n <= 24 else (128 if n <= 64 else 256)
+    return 0
 
 
 def table() -> dict:
@@ -58,6 +59,8 @@ def tune(make_batch, mode: str, n_max: int, n_pieces: int, iters: int = 20, save
     import torch
     res = {}
     for blk in BLOCKS:
+        if blk == 1 and n_max > WAVE_MAX_TREES:
+            continue
         b = make_batch()
         b.opts.block = blk
         b.launch(1)                      # first evaluation + one iteration (warm)

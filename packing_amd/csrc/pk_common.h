@@ -144,6 +144,88 @@ __device__ inline double wave_max(double v) {
     return fmax(fmax(lane_d(v, 0), lane_d(v, 16)), fmax(lane_d(v, 32), lane_d(v, 48)));
 }
 
+__device__ inline double shfl_xor_d(double v, int mask) {
+    const int lo = __shfl_xor(__double2loint(v), mask);
+    const int hi = __shfl_xor(__double2hiint(v), mask);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double shfl_d(double v, int src) {
+    const int lo = __shfl(__double2loint(v), src);
+    const int hi = __shfl(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float lane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+constexpr int ilog2c(int k) { return k <= 1 ? 0 : 1 + ilog2c(k / 2); }
+
+// (send, keep) = up ? (lo, hi) : (hi, lo), as a bitwise blend: a select between two
+// elements of a register array can be rewritten by the compiler into a dynamically
+// indexed load, which moves the whole array to scratch memory.
+__device__ inline void blend_pair(double lo, double hi, bool up, double& send, double& keep) {
+    const unsigned long long m = up ? ~0ull : 0ull;
+    const unsigned long long a = __double_as_longlong(lo), b = __double_as_longlong(hi);
+    send = __longlong_as_double((a & m) | (b & ~m));
+    keep = __longlong_as_double((b & m) | (a & ~m));
+}
+
+// Sums of K values (K a power of two, 2..64) over the 64 lanes of a wave in one
+// butterfly.  At stage s the lanes l and l ^ (32 >> s) exchange halves of their value
+// lists and add, so after log2 K stages lane l holds value l >> (6 - log2 K); the
+// remaining stages add it across the other lanes.  Every value is summed by the same
+// pairing tree (pairs differing in lane bit 5, then 4, ..., 0), and each pair sum is
+// symmetric, so a value's total does not depend on K or on its position in the list.
+// The full wave must be active.  Returns this lane's value.
+template <int K>
+__device__ inline double wave_sum_many(double (&v)[K]) {
+    constexpr int LK = ilog2c(K);
+    static_assert((1 << LK) == K && K >= 2 && K <= 64, "K must be a power of two in [2, 64]");
+    const int lane = __lane_id();
+#pragma unroll
+    for (int s = 0; s < LK; ++s) {
+        const int bit = 32 >> s;
+        const int h = K >> (s + 1);
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            double send, keep;
+            blend_pair(v[j], v[h + j], up, send, keep);
+            v[j] = keep + shfl_xor_d(send, bit);
+        }
+    }
+    double r = v[0];
+#pragma unroll
+    for (int s = LK; s < 6; ++s) r += shfl_xor_d(r, 32 >> s);
+    return r;
+}
+// The same layout with max instead of sum (exact in any order).
+template <int K>
+__device__ inline double wave_max_many(double (&v)[K]) {
+    constexpr int LK = ilog2c(K);
+    static_assert((1 << LK) == K && K >= 2 && K <= 64, "K must be a power of two in [2, 64]");
+    const int lane = __lane_id();
+#pragma unroll
+    for (int s = 0; s < LK; ++s) {
+        const int bit = 32 >> s;
+        const int h = K >> (s + 1);
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            double send, keep;
+            blend_pair(v[j], v[h + j], up, send, keep);
+            v[j] = fmax(keep, shfl_xor_d(send, bit));
+        }
+    }
+    double r = v[0];
+#pragma unroll
+    for (int s = LK; s < 6; ++s) r = fmax(r, shfl_xor_d(r, 32 >> s));
+    return r;
+}
+// Lane holding value j of a K-value butterfly result.
+template <int K>
+__device__ inline int many_src(int j) { return j << (6 - ilog2c(K)); }
+
 __device__ inline uint64_t lanemask_lt() {
     int lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));

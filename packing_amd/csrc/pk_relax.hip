@@ -42,53 +42,54 @@ struct RelaxArgs {
     LutDev lut;
 };
 
-struct F2 {
-    float a, b;
-};
-struct F3 {
-    float a, b, c;
-};
-
-// Wave-0 vector loop over k = lane + 64 r, r = 0 .. np/64 - 1 increasing (per-lane sums keep
-// a fixed order), in blocks of WB elements per lane whose loads are issued before any of
-// the block's stores: load(k) -> value, then op(k, value).  np is a multiple of 64; the
-// vectors are zero beyond their true length.
-constexpr int WB = 8;
-template <typename LoadF, typename OpF>
-__device__ inline void w0_for(int np, LoadF load, OpF op) {
-    const int lane = threadIdx.x;
-    const int R = np / WAVE;
-    using V = decltype(load(0));
-    for (int r0 = 0; r0 < R; r0 += WB) {
-        V v[WB];
-#pragma unroll
-        for (int u = 0; u < WB; ++u)
-            if (r0 + u < R) v[u] = load(lane + WAVE * (r0 + u));
-#pragma unroll
-        for (int u = 0; u < WB; ++u)
-            if (r0 + u < R) op(lane + WAVE * (r0 + u), v[u]);
-    }
-}
-
 __host__ __device__ inline int pad64(int n) { return (n + WAVE - 1) / WAVE * WAVE; }
 
 // Cost and gradient of the state x into g (all threads); returns the loss.  list_mode:
-// 1 rebuilds the neighbour list while evaluating, 2 evaluates from it.
+// 1 rebuilds the neighbour list while evaluating, 2 evaluates from it (no cell grid).
+// Each tree's poses, sincos and boundary term are handled by the same thread, so the
+// staging needs no barrier between them.
 template <int MODE, int BLOCK>
-__device__ inline double relax_eval(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
+__device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
                                     const float* x, float* g, int N,
                                     unsigned long long* bacc /* [4] boundary accumulators */,
                                     long long& st_tp, long long& st_pp, int list_mode, float cut_list) {
     const pk_relax_params& p = A.p;
+    const bool grid = list_mode != 2;
     for (int i = threadIdx.x; i < N; i += BLOCK) {
+        const float ti = x[3 * i + 2];
         L.x[i] = x[3 * i];
         L.y[i] = x[3 * i + 1];
-        L.t[i] = x[3 * i + 2];
+        L.t[i] = ti;
+        float sn, cs;
+        sincos_t(ti, &sn, &cs);
+        L.c[i] = cs;
+        L.s[i] = sn;
     }
+    for (int q = threadIdx.x; q < 3 * N + 1; q += BLOCK) L.g[q] = 0ull;
+    if (grid)
+        for (int h = threadIdx.x; h < L.H; h += BLOCK) L.head[h] = -1;
     if (threadIdx.x < 4) bacc[threadIdx.x] = 0ull;
     if (threadIdx.x == 0 && list_mode == 1) L.wcnt[8] = 0;
     __syncthreads();
-    stage_instance<float, BLOCK, MODE != MODE_LUT, true>(G, L);
+    if (MODE != MODE_LUT) {
+        const int P = L.P, nv = N * P * MAXV;
+        for (int q = threadIdx.x; q < nv; q += BLOCK) {
+            const int i = q / (P * MAXV);
+            const int r = q - i * P * MAXV;
+            const float lx = G.px[r], ly = G.py[r];
+            L.vx[q] = L.c[i] * lx - L.s[i] * ly;
+            L.vy[q] = L.s[i] * lx + L.c[i] * ly;
+        }
+    }
+    if (grid) {
+        const float inv = grid_inv_side(G);
+        for (int i = threadIdx.x; i < N; i += BLOCK) {
+            const uint32_t key = cell_key(L.x[i] * inv, L.y[i] * inv);
+            L.cell[i] = key;
+            L.next[i] = atomicExch(&L.head[cell_bucket(key, L.H)], i);
+        }
+    }
+    if (MODE != MODE_LUT || grid) __syncthreads();
     Evaluator<float, MODE, true, false, BLOCK> ev(G, L);
     ev.cax = ev.cay = ev.cbx = ev.cby = 0.f;
     ev.npd = 0;
@@ -100,20 +101,27 @@ __device__ inline double relax_eval(const GeomT<float>& G, EvalLds<float>& L, co
     st_tp += ev.n_tp;
     st_pp += ev.n_pp;
     const float half = x[3 * N] * 0.5f, hx = x[3 * N + 1], hy = x[3 * N + 2];
+    unsigned long long lb[4] = {0ull, 0ull, 0ull, 0ull};
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         float gb[3], gh[3];
         const float cb = boundary_tree<float>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
-        Acc<float>::add(&bacc[0], cb);
-        Acc<float>::add(&bacc[1], gh[0]);
-        Acc<float>::add(&bacc[2], gh[1]);
-        Acc<float>::add(&bacc[3], gh[2]);
+        lb[0] += Acc<float>::local(cb);
+        lb[1] += Acc<float>::local(gh[0]);
+        lb[2] += Acc<float>::local(gh[1]);
+        lb[3] += Acc<float>::local(gh[2]);
 #pragma unroll
         for (int k = 0; k < 3; ++k)
             g[3 * i + k] = p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * gb[k];
     }
+    // integer sums: exact in any order
+    if (__ballot((lb[0] | lb[1] | lb[2] | lb[3]) != 0ull)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) Acc<float>::add_raw(&bacc[k], lb[k]);
+    }
     __syncthreads();
     const double c_ov = Acc<float>::get(*L.cost), c_bd = Acc<float>::get(bacc[0]);
     double loss = (double)p.w_overlap * c_ov + (double)p.w_boundary * c_bd;
+    // the h gradient is read by wave 0, which thread 0 belongs to, after this returns
     if (threadIdx.x == 0) {
         if (p.fixed_h) {
             g[3 * N] = g[3 * N + 1] = g[3 * N + 2] = 0.f;
@@ -124,30 +132,288 @@ __device__ inline double relax_eval(const GeomT<float>& G, EvalLds<float>& L, co
         }
     }
     if (!p.fixed_h) loss += (double)p.w_area * (double)x[3 * N] * (double)x[3 * N];
-    __syncthreads();
     return loss;
 }
 
-// Dynamic LDS of one workgroup: evaluator, x g g_prev d, and (hl) the history s, y; the
-// vectors padded to a multiple of 64 floats.
+// Dynamic LDS of one workgroup: evaluator, x g g_prev d, the small L-BFGS matrices S^T Y and
+// Y^T Y (history x history doubles each), and (hl) the history s, y; the vectors padded to a
+// multiple of 64 floats.
 size_t relax_lds_bytes(int mode, int n_max, int P, int m, bool hl, int block) {
     const int Pe = mode == MODE_LUT ? 0 : P;
     size_t b = eval_lds_bytes<float>(n_max, Pe, false, block / WAVE, true);
     const size_t np = (size_t)pad64(3 * n_max + 3);
     b += sizeof(float) * 4 * np;
+    b += sizeof(double) * 2 * (size_t)m * m;
+    b = align_up(b, 16);
     if (hl) b += sizeof(float) * 2 * (size_t)m * np;
     return align_up(b, 16);
 }
 
+// History rows: physical slot offsets into s and y (row stride np in LDS, n in global
+// memory, where elements beyond n belong to the next row and read as zero).
+template <bool HL>
+struct Hist {
+    float* s;
+    float* y;
+    int stride, n;
+    __device__ inline float S(int off, int k) const { return (HL || k < n) ? s[off + k] : 0.f; }
+    __device__ inline float Y(int off, int k) const { return (HL || k < n) ? y[off + k] : 0.f; }
+};
+
+// ---------------------------------------------------------------------------------
+// L-BFGS arithmetic shared by both relaxation kernels.  Vectors are walked element by
+// element, k = lane + 64 r with r increasing; per-lane partial sums are double and every
+// wave-wide sum goes through wave_sum_many, whose result does not depend on the list it
+// sits in.  Both kernels therefore produce the same bits, wherever the vectors live.
+// History rows are passed in chronological order (oldest first); rows at or beyond the
+// used count are ignored.
+// ---------------------------------------------------------------------------------
+
+constexpr int pow2_ceil(int k) { return k <= 1 ? 1 : 2 * pow2_ceil((k + 1) / 2); }
+
+// Ring slot of chronological row i: (head + i) mod m for 0 <= head, i < m, without an
+// integer division.
+__host__ __device__ inline int ring(int head, int i, int m) {
+    const int v = head + i;
+    return v >= m ? v - m : v;
+}
+
+template <int KH>
+struct Lb {
+    static constexpr int KD = 2 * KH;             // dots: i = s_i.v, KH + i = y_i.v
+    static constexpr int KA = pow2_ceil(2 + KH);  // update A: 0 = y.s, 1 = y.y, 2 + i = s_i.y
+    static constexpr int KB = KH;                 // update B: i = y_i.y
+
+    __device__ static inline void dots(double (&acc)[KD], const float (&sv)[KH], const float (&yv)[KH],
+                                       float v, int nv) {
+        const double vd = (double)v;
+#pragma unroll
+        for (int i = 0; i < KH; ++i) {
+            if (i < nv) {
+                acc[i] += (double)sv[i] * vd;
+                acc[KH + i] += (double)yv[i] * vd;
+            }
+        }
+    }
+    // d_k = -gamma g_k + sum_i (cs_i s_i,k + cy_i y_i,k), in float, in this order
+    __device__ static inline float dir(float gamf, float gk, const float (&cs)[KH], const float (&cy)[KH],
+                                       const float (&sv)[KH], const float (&yv)[KH], int hl) {
+        float dk = -gamf * gk;
+#pragma unroll
+        for (int i = 0; i < KH; ++i) {
+            if (i < hl) {
+                dk += cs[i] * sv[i];
+                dk += cy[i] * yv[i];
+            }
+        }
+        return dk;
+    }
+    __device__ static inline void upd(double (&a)[KA], double (&b)[KB], float yk, float sk,
+                                      const float (&sv)[KH], const float (&yv)[KH], int nv) {
+        const double yd = (double)yk;
+        a[0] += yd * (double)sk;
+        a[1] += yd * yd;
+#pragma unroll
+        for (int i = 0; i < KH; ++i) {
+            if (i < nv) {
+                a[2 + i] += (double)sv[i] * yd;
+                b[i] += (double)yv[i] * yd;
+            }
+        }
+    }
+
+    // Direction coefficients of the compact L-BFGS form (Byrd, Nocedal and Schnabel 1994,
+    // eq. 3.1): with a = S^T g, b = Y^T g (r: the dots butterfly), R = upper part of S^T Y,
+    // D = diag(S^T Y), gamma = H_diag:
+    //   u = R^-1 a,  w = R^-T ((D + gamma Y^T Y) u - gamma b),  H g = gamma g + S w - gamma Y u,
+    // the same direction as the two-loop recursion of core/lbfgs_torch_parallel.py:532-572.
+    // The small matrices are indexed by ring slot, (head + i) mod m for chronological i.  The
+    // hl x hl triangular solves run one row per lane.  Returns, in lane i < hl, the
+    // coefficients of d = -gamma g + sum_i cs_i s_i + cy_i y_i.
+    __device__ static inline void solve(double r, int hl, int head, int m, double gam, const double* SY,
+                                        const double* YY, const double* ro, float& cs_l, float& cy_l) {
+        const int lane = __lane_id();
+        const double a_l = shfl_d(r, many_src<KD>(lane & (KH - 1)));
+        const double b_l = shfl_d(r, many_src<KD>(KH + (lane & (KH - 1))));
+        const bool row = lane < hl;
+        const int pl = ring(head, lane < m ? lane : 0, m);
+        double acc = a_l, u_l = 0.0, zacc = 0.0;
+        for (int j = hl - 1; j >= 0; --j) {
+            const int pj = ring(head, j, m);
+            const double uj = lane_d(acc, j) * ro[pj];
+            if (lane == j) u_l = uj;
+            if (lane < j) acc -= SY[pl * m + pj] * uj;
+            if (row) zacc += YY[pl * m + pj] * uj;
+        }
+        const double z_l = row ? SY[pl * m + pl] * u_l + gam * zacc - gam * b_l : 0.0;
+        double accw = z_l, w_l = 0.0;
+        for (int j = 0; j < hl; ++j) {
+            const int pj = ring(head, j, m);
+            const double wj = lane_d(accw, j) * ro[pj];
+            if (lane == j) w_l = wj;
+            if (row && lane > j) accw -= SY[pj * m + pl] * wj;
+        }
+        cs_l = row ? (float)(-w_l) : 0.f;
+        cy_l = row ? (float)(gam * u_l) : 0.f;
+    }
+
+    // Accept the pair into ring slot q: new column of S^T Y and row of Y^T Y (ra, rb: the
+    // update butterflies over the hl rows in use; a dropped oldest row is harmless),
+    // H_diag, ro and the ring counters.
+    __device__ static inline void store(double ra, double rb, double ys, double yy, int hl, int head, int m,
+                                        int q, double* SY, double* YY, LbSlot& S) {
+        const int lane = __lane_id();
+        const double syc = shfl_d(ra, many_src<KA>(2 + (lane & (KH - 1))));
+        const double yyr = shfl_d(rb, many_src<KB>(lane & (KH - 1)));
+        if (lane < hl) {
+            const int pi = ring(head, lane < m ? lane : 0, m);
+            if (pi != q) {
+                SY[pi * m + q] = syc;
+                YY[pi * m + q] = yyr;
+                YY[q * m + pi] = yyr;
+            }
+        }
+        if (lane == 0) {
+            SY[q * m + q] = ys;
+            YY[q * m + q] = yy;
+            S.H = ys / yy;
+            S.ro[q] = 1.0 / ys;
+            if (S.hist_len < m) S.hist_len++;
+            else S.head = ring(S.head, 1, m);
+        }
+    }
+
+    // S^T Y (upper) and Y^T Y for column j (chronological) from the dots butterfly of y_j.
+    __device__ static inline void rebuild_col(double r, int j, int head, int m, double* SY, double* YY) {
+        const int lane = __lane_id();
+        const double sy = shfl_d(r, many_src<KD>(lane & (KH - 1)));
+        const double yy = shfl_d(r, many_src<KD>(KH + (lane & (KH - 1))));
+        if (lane <= j) {
+            const int pi = ring(head, lane < m ? lane : 0, m), pj = ring(head, j, m);
+            SY[pi * m + pj] = sy;
+            YY[pi * m + pj] = yy;
+            YY[pj * m + pi] = yy;
+        }
+    }
+};
+
+// History rows of element k for the first nv rows (chronological), from LDS or global memory.
+template <int KH, bool HL>
+__device__ inline void hist_elem(const Hist<HL>& H, const int (&off)[KH], int k, int nv, float (&sv)[KH],
+                                 float (&yv)[KH]) {
+#pragma unroll
+    for (int i = 0; i < KH; ++i) {
+        sv[i] = i < nv ? H.S(off[i], k) : 0.f;
+        yv[i] = i < nv ? H.Y(off[i], k) : 0.f;
+    }
+}
+template <int KH, bool HL>
+__device__ inline void hist_offsets(const Hist<HL>& H, int head, int m, int (&off)[KH]) {
+#pragma unroll
+    for (int i = 0; i < KH; ++i) off[i] = ring(head, i < m ? i : 0, m) * H.stride;
+}
+
+// Dots of v with the first nv history rows (LDS / global history).  Wave 0 only.
+template <int KH, bool HL, typename VF>
+__device__ inline double lb_dots(int np, int nv, int head, int m, const Hist<HL>& H, VF v) {
+    double acc[Lb<KH>::KD];
+#pragma unroll
+    for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.0;
+    int off[KH];
+    hist_offsets<KH, HL>(H, head, m, off);
+    for (int k = threadIdx.x; k < np; k += WAVE) {
+        float sv[KH], yv[KH];
+        hist_elem<KH, HL>(H, off, k, nv, sv, yv);
+        Lb<KH>::dots(acc, sv, yv, v(k), nv);
+    }
+    return wave_sum_many<Lb<KH>::KD>(acc);
+}
+
+// Rebuild S^T Y and Y^T Y from the stored history rows (same sums as the incremental
+// updates, so a call that starts from stored history continues bit for bit).  Wave 0 only.
+template <int KH, bool HL>
+__device__ inline void lb_rebuild(int np, int hl, int head, int m, const Hist<HL>& H, double* SY,
+                                  double* YY) {
+    for (int j = 0; j < hl; ++j) {
+        const int oj = ring(head, j, m) * H.stride;
+        const double r = lb_dots<KH, HL>(np, j + 1, head, m, H, [&](int k) { return H.Y(oj, k); });
+        Lb<KH>::rebuild_col(r, j, head, m, SY, YY);
+    }
+}
+
+// Direction pass: d written to LDS, with g.d and sum |g| (one butterfly), max |d| and
+// max |x| (another).  Wave 0 only.
+template <int KH, bool HL>
+__device__ inline void lb_direction(int np, int hl, int head, int m, float gamf, const Hist<HL>& H,
+                                    const float* g, const float* x, float* d, float cs_l, float cy_l,
+                                    double& gtd, double& gsum, double& dmax, double& xmax) {
+    float cs[KH], cy[KH];
+#pragma unroll
+    for (int i = 0; i < KH; ++i) {
+        cs[i] = lane_f(cs_l, i);
+        cy[i] = lane_f(cy_l, i);
+    }
+    int off[KH];
+    hist_offsets<KH, HL>(H, head, m, off);
+    double sv2[2] = {0.0, 0.0}, mv[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < np; k += WAVE) {
+        float sv[KH], yv[KH];
+        hist_elem<KH, HL>(H, off, k, hl, sv, yv);
+        const float gk = g[k];
+        const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv, hl);
+        d[k] = dk;
+        sv2[0] += (double)gk * (double)dk;
+        sv2[1] += fabs((double)gk);
+        mv[0] = fmax(mv[0], fabs((double)dk));
+        mv[1] = fmax(mv[1], fabs((double)x[k]));
+    }
+    const double rs = wave_sum_many<2>(sv2);
+    const double rm = wave_max_many<2>(mv);
+    gtd = lane_d(rs, many_src<2>(0));
+    gsum = lane_d(rs, many_src<2>(1));
+    dmax = lane_d(rm, many_src<2>(0));
+    xmax = lane_d(rm, many_src<2>(1));
+}
+
+// Update pass after an evaluation: with y = g - g_prev and s = t d, the sums y.s, y.y,
+// s_i.y (ra) and y_i.y (rb) for the nv history rows in use, and max |g|, max |s|.  Wave 0.
+template <int KH, bool HL>
+__device__ inline void lb_update_dots(int np, int nv, int head, int m, float tf, const Hist<HL>& H,
+                                      const float* g, const float* gp, const float* d, double& ra,
+                                      double& rb, double& gmax, double& smax) {
+    double a[Lb<KH>::KA], bb[Lb<KH>::KB];
+#pragma unroll
+    for (int i = 0; i < Lb<KH>::KA; ++i) a[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < Lb<KH>::KB; ++i) bb[i] = 0.0;
+    int off[KH];
+    hist_offsets<KH, HL>(H, head, m, off);
+    double mv[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < np; k += WAVE) {
+        float sv[KH], yv[KH];
+        hist_elem<KH, HL>(H, off, k, nv, sv, yv);
+        const float gk = g[k];
+        const float yk = gk - gp[k];
+        const float sk = d[k] * tf;
+        Lb<KH>::upd(a, bb, yk, sk, sv, yv, nv);
+        mv[0] = fmax(mv[0], fabs((double)gk));
+        mv[1] = fmax(mv[1], fabs((double)sk));
+    }
+    const double rm = wave_max_many<2>(mv);
+    gmax = lane_d(rm, many_src<2>(0));
+    smax = lane_d(rm, many_src<2>(1));
+    ra = wave_sum_many<Lb<KH>::KA>(a);
+    rb = wave_sum_many<Lb<KH>::KB>(bb);
+}
+
 // HL: the history s_hist / y_hist is copied into LDS at the start of the call and back at
-// the end; otherwise the two-loop recursion reads it from global memory (row stride n
-// there, padded stride np in LDS).
-template <int MODE, int BLOCK, bool HL>
+// the end; otherwise the L-BFGS passes read it from global memory (row stride n there,
+// padded stride np in LDS).  KH: history rows the unrolled passes handle (history <= KH).
+template <int MODE, int BLOCK, bool HL, int KH>
 __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     extern __shared__ __align__(16) char smem[];
     __shared__ GeomT<float> G;
     __shared__ LbSlot S;
-    __shared__ double al[PK_MAX_HISTORY];
     __shared__ unsigned long long bacc[4];
     __shared__ int stop;
     __shared__ double t_step;
@@ -167,7 +433,6 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     if (S.status == ST_DONE) return;
 
     const int n = 3 * N + 3, np = pad64(n), m = p.history;
-    const int hstride = HL ? np : n;   // history row stride
     const long long xoff = p.xoff[slot];
     float* __restrict__ X = p.x + xoff;
     float* __restrict__ Gg = p.g + xoff;
@@ -181,11 +446,18 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     float* g = x + np;
     float* gp = g + np;
     float* d = gp + np;
-    float* hs = d + np;          // LDS history (HL): [m][np] s, then [m][np] y
-    float* hy = hs + m * np;
-    if (!HL) {
-        hs = Sh;
-        hy = Yh;
+    double* SY = reinterpret_cast<double*>(d + np);   // [m][m] s_a . y_b by physical slot
+    double* YY = SY + m * m;                            // [m][m] y_a . y_b
+    Hist<HL> H;
+    H.n = n;
+    if (HL) {
+        H.s = reinterpret_cast<float*>(reinterpret_cast<char*>(SY) + align_up(sizeof(double) * 2 * m * m, 16));
+        H.y = H.s + m * np;
+        H.stride = np;
+    } else {
+        H.s = Sh;
+        H.y = Yh;
+        H.stride = n;
     }
     for (int k = threadIdx.x; k < np; k += BLOCK) {
         const bool in = k < n;
@@ -199,16 +471,11 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
         for (int q = threadIdx.x; q < m * np; q += BLOCK) {
             const int r = q / np, k = q - r * np;
             const bool in = run && k < n;
-            hs[q] = in ? Sh[(long long)r * n + k] : 0.f;
-            hy[q] = in ? Yh[(long long)r * n + k] : 0.f;
+            H.s[q] = in ? Sh[(long long)r * n + k] : 0.f;
+            H.y[q] = in ? Yh[(long long)r * n + k] : 0.f;
         }
     }
     __syncthreads();
-    // history element k of row ph (zero beyond n)
-    auto hrow = [&](const float* base, int ph, int k) -> float {
-        if (HL) return base[ph * hstride + k];
-        return k < n ? base[ph * hstride + k] : 0.f;
-    };
     long long st_tp = 0, st_pp = 0;
     const bool w0 = threadIdx.x < WAVE;
     const int lane = threadIdx.x;
@@ -221,13 +488,14 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
         nb_valid = 0;
         vdisp = 0.0;
     }
+    if (w0 && S.status == ST_RUN && S.hist_len > 0) lb_rebuild<KH, HL>(np, S.hist_len, S.head, m, H, SY, YY);
 
     if (S.status == ST_FRESH) {
         const double loss = relax_eval<MODE, BLOCK>(G, L, A, x, g, N, bacc, st_tp, st_pp, 1, cut_list);
         if (w0) {
             if (lane == 0) nb_valid = L.wcnt[8] <= L.nb_cap;
             double gm = 0.0;
-            w0_for(np, [&](int k) { return g[k]; }, [&](int, float v) { gm = fmax(gm, fabs((double)v)); });
+            for (int k = lane; k < np; k += WAVE) gm = fmax(gm, fabs((double)g[k]));
             gm = wave_max(gm);
             if (lane == 0) {
                 S.loss = loss;
@@ -247,108 +515,39 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
 
     for (int it = 0; it < p.iters; ++it) {
         if (S.status == ST_DONE) break;
-        // ---- direction, step length, directional-derivative test (wave 0) ----
-        // Two-loop recursion with each axpy fused with the next dot product (same values
-        // and summation order as separate passes):
-        //   q = -g; for i = hl-1..0: a_i = ro_i s_i.q, q -= a_i y_i;  r = H q;
-        //   for i = 0..hl-1: b_i = ro_i y_i.r, r += (a_i - b_i) s_i;  d = r.
+        // ---- direction, step length, directional-derivative test, step (wave 0) ----
         if (w0) {
             const int niter = S.n_iter + 1;
             const int hl = niter > 1 ? S.hist_len : 0;
-            const float hd = (float)S.H;
             const int head = S.head;
-            double acc = 0.0;
+            float cs_l = 0.f, cy_l = 0.f;
             if (hl > 0) {
-                const int ps = (head + hl - 1) % m;
-                w0_for(np, [&](int k) { return F2{g[k], hrow(hs, ps, k)}; }, [&](int k, F2 v) {
-                    const float q = -v.a;
-                    d[k] = q;
-                    acc += (double)v.b * (double)q;
-                });
-            } else if (niter > 1) {
-                w0_for(np, [&](int k) { return g[k]; }, [&](int k, float v) { d[k] = (-v) * hd; });
-            } else {
-                w0_for(np, [&](int k) { return g[k]; }, [&](int k, float v) { d[k] = -v; });
+                const double r = lb_dots<KH, HL>(np, hl, head, m, H, [&](int k) { return g[k]; });
+                Lb<KH>::solve(r, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
             }
-            for (int i = hl - 1; i >= 0; --i) {
-                const int ph = (head + i) % m;
-                const double a_i = wave_sum(acc) * S.ro[ph];
-                al[i] = a_i;   // every lane writes the same value and reads its own write
-                const float af = (float)a_i;
-                acc = 0.0;
-                if (i > 0) {
-                    const int pn = (head + i - 1) % m;
-                    w0_for(np, [&](int k) { return F3{d[k], hrow(hy, ph, k), hrow(hs, pn, k)}; },
-                           [&](int k, F3 v) {
-                               const float q = v.a - v.b * af;
-                               d[k] = q;
-                               acc += (double)v.c * (double)q;
-                           });
-                } else {
-                    // last step of the first loop, the H scaling and the first dot of the
-                    // second loop (y of slot i = 0 both times)
-                    w0_for(np, [&](int k) { return F2{d[k], hrow(hy, ph, k)}; }, [&](int k, F2 v) {
-                        const float q = v.a - v.b * af;
-                        const float r = q * hd;
-                        d[k] = r;
-                        acc += (double)v.b * (double)r;
-                    });
-                }
-            }
-            for (int i = 0; i < hl; ++i) {
-                const int ph = (head + i) % m;
-                const double be = wave_sum(acc) * S.ro[ph];
-                const float c = (float)(al[i] - be);
-                acc = 0.0;
-                if (i + 1 < hl) {
-                    const int pn = (head + i + 1) % m;
-                    w0_for(np, [&](int k) { return F3{d[k], hrow(hs, ph, k), hrow(hy, pn, k)}; },
-                           [&](int k, F3 v) {
-                               const float r = v.a + v.b * c;
-                               d[k] = r;
-                               acc += (double)v.c * (double)r;
-                           });
-                } else {
-                    w0_for(np, [&](int k) { return F2{d[k], hrow(hs, ph, k)}; },
-                           [&](int k, F2 v) { d[k] = v.a + v.b * c; });
-                }
-            }
-            double t;
-            if (niter == 1) {
-                double gs = 0.0;
-                w0_for(np, [&](int k) { return g[k]; }, [&](int, float v) { gs += fabs((double)v); });
-                gs = wave_sum(gs);
-                t = fmin(1.0, 1.0 / gs) * (double)p.lr;
-            } else {
-                t = (double)p.lr;
-            }
-            // step clamp and directional derivative in one pass
-            const double t_pre = t;
-            double pc = 0.0, gtd = 0.0;
-            w0_for(np, [&](int k) { return F2{g[k], d[k]}; }, [&](int, F2 v) {
-                pc = fmax(pc, fabs((double)v.b * t));
-                gtd += (double)v.a * (double)v.b;
-            });
-            gtd = wave_sum(gtd);
-            pc = wave_max(pc);
+            // iteration 1: d = -g; later: d = -H_diag g + history terms
+            const float gamf = niter > 1 ? (float)S.H : 1.f;
+            double gtd, gsum, dmax, xmax;
+            lb_direction<KH, HL>(np, hl, head, m, gamf, H, g, x, d, cs_l, cy_l, gtd, gsum, dmax, xmax);
+            double t = niter == 1 ? fmin(1.0, 1.0 / gsum) * (double)p.lr : (double)p.lr;
+            // step clamp: max |t d| <= max_step
+            const double pc = dmax * fabs(t);
             if (p.max_step > 0.f && pc > (double)p.max_step) t *= (double)p.max_step / pc;
             const bool es = gtd > -(double)p.tol_change;
-            double xm = 0.0;
             if (!es) {
                 const float tf = (float)t;
-                w0_for(np, [&](int k) { return F3{x[k], d[k], g[k]}; }, [&](int k, F3 v) {
-                    gp[k] = v.c;
-                    const float xn = v.a + tf * v.b;
-                    x[k] = xn;
-                    xm = fmax(xm, fabs((double)xn));
-                });
-                xm = wave_max(xm);
+                for (int k = lane; k < np; k += WAVE) {
+                    const float dk = d[k];
+                    gp[k] = g[k];
+                    x[k] = x[k] + tf * dk;
+                }
             }
             if (lane == 0) {
-                // largest coordinate change: pc scaled to the final t, plus the rounding of
-                // the float update (half an ulp of the largest coordinate, doubled); a
-                // centre moves by at most sqrt(2) times that
-                const double step = pc * (t / t_pre);
+                // largest coordinate change |t d|, plus the rounding of the float update (half an
+                // ulp of the largest coordinate, doubled); a centre moves by at most sqrt(2)
+                // times that
+                const double step = dmax * fabs(t);
+                const double xm = xmax + step;
                 vdisp += 1.4142136 * (step * (1.0 + 1e-6) + xm * 1.2e-7);
                 eval_mode = (nb_valid && 2.0 * vdisp < 0.9 * (double)skin) ? 2 : 1;
                 if (eval_mode == 1) vdisp = 0.0;
@@ -377,54 +576,41 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
             const double t = t_step;
             const float tf = (float)t;
             const int niter = S.n_iter;
-            double ys = 0.0, yy = 0.0, gm = 0.0, pc = 0.0;
-            w0_for(np, [&](int k) { return F3{g[k], gp[k], d[k]}; }, [&](int, F3 v) {
-                const double yk = (double)(v.a - v.b);
-                const double sk = (double)(v.c * tf);
-                ys += yk * sk;
-                yy += yk * yk;
-                gm = fmax(gm, fabs((double)v.a));
-                pc = fmax(pc, fabs(sk));
-            });
-            gm = wave_max(gm);
-            pc = wave_max(pc);
-            if (niter > 1) {
-                ys = wave_sum(ys);
-                yy = wave_sum(yy);
-                if (ys > 1e-10) {
-                    const int slotp = S.hist_len < m ? (S.head + S.hist_len) % m : S.head;
-                    float* so = hs + slotp * hstride;
-                    float* yo = hy + slotp * hstride;
-                    w0_for(np, [&](int k) { return F3{g[k], gp[k], d[k]}; }, [&](int k, F3 v) {
-                        if (HL || k < n) {
-                            yo[k] = v.a - v.b;
-                            so[k] = v.c * tf;
-                        }
-                    });
-                    if (lane == 0) {
-                        S.H = ys / yy;
-                        S.ro[slotp] = 1.0 / ys;
-                        if (S.hist_len < m) S.hist_len++;
-                        else S.head = (S.head + 1) % m;
+            const int hl = S.hist_len, head = S.head;
+            // history pairs are stored from iteration 2 on (core/lbfgs_torch_parallel.py:636-690)
+            const int nv = niter > 1 ? hl : 0;
+            double gm, pc, ra, rb;
+            lb_update_dots<KH, HL>(np, nv, head, m, tf, H, g, gp, d, ra, rb, gm, pc);
+            const double ys = lane_d(ra, many_src<Lb<KH>::KA>(0));
+            const double yy = lane_d(ra, many_src<Lb<KH>::KA>(1));
+            if (niter > 1 && ys > 1e-10) {
+                const int q = hl < m ? ring(head, hl, m) : head;
+                const int oq = q * H.stride;
+                for (int k = lane; k < np; k += WAVE) {
+                    if (HL || k < n) {
+                        const float gk = g[k];
+                        H.y[oq + k] = gk - gp[k];
+                        H.s[oq + k] = d[k] * tf;
                     }
                 }
+                Lb<KH>::store(ra, rb, ys, yy, hl, head, m, q, SY, YY, S);
             }
             if (lane == 0) {
                 S.n_eval += 1;
                 S.loss = loss;
-                int r = 0;
-                if (gm <= (double)p.tol_grad) r |= R_GRAD;
-                if (S.n_eval >= p.max_eval) r |= R_EVAL;
-                if (pc <= (double)p.tol_change) r |= R_PARAM;
+                int rr = 0;
+                if (gm <= (double)p.tol_grad) rr |= R_GRAD;
+                if (S.n_eval >= p.max_eval) rr |= R_EVAL;
+                if (pc <= (double)p.tol_change) rr |= R_PARAM;
                 const double lc = fabs(loss - loss_prev);
-                if (lc < (double)p.tol_change) r |= R_LOSS;
+                if (lc < (double)p.tol_change) rr |= R_LOSS;
                 if (p.tol_rel > 0.f && niter > m && lc / (fabs(loss_prev) + 1e-12) < (double)p.tol_rel)
-                    r |= R_REL;
-                if (p.stop_on_increase && loss > S.loss0) r |= R_INCREASE;
-                if (niter >= p.max_iter) r |= R_ITER;
-                if (r) {
+                    rr |= R_REL;
+                if (p.stop_on_increase && loss > S.loss0) rr |= R_INCREASE;
+                if (niter >= p.max_iter) rr |= R_ITER;
+                if (rr) {
                     S.status = ST_DONE;
-                    S.reason |= r;
+                    S.reason |= rr;
                 }
             }
         }
@@ -438,8 +624,8 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     if (HL) {
         for (int q = threadIdx.x; q < m * n; q += BLOCK) {
             const int r = q / n, k = q - r * n;
-            Sh[q] = hs[r * np + k];
-            Yh[q] = hy[r * np + k];
+            Sh[q] = H.s[r * np + k];
+            Yh[q] = H.y[r * np + k];
         }
     }
     if (threadIdx.x == 0) {
@@ -453,12 +639,359 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     }
 }
 
+// Evaluation of the register state x into g for k_relax_w (through the LDS vectors xs, gs).
+template <int MODE, int R>
+__device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
+                                                const float (&x)[R], float (&g)[R], float* xs, float* gs,
+                                                int N, int n, unsigned long long* bacc, long long& st_tp,
+                                                long long& st_pp, int lm, float cut_list, int& nb_valid) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < R; ++r) xs[lane + WAVE * r] = x[r];
+    __syncthreads();
+    const double loss = relax_eval<MODE, WAVE>(G, L, A, xs, gs, N, bacc, st_tp, st_pp, lm, cut_list);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = lane + WAVE * r;
+        g[r] = k < n ? gs[k] : 0.f;
+    }
+    if (lm == 1) nb_valid = L.wcnt[8] <= L.nb_cap;
+    return loss;
+}
+
+// One wave per instance, with the state vectors (x, g, g_prev, d) and the L-BFGS history
+// in registers: element k of a state vector is row k / 64 of lane k % 64 (R rows cover
+// 64 R >= 3N + 3 elements; rows beyond the state are zero), history rows in chronological
+// order (oldest first, shifted down when the ring is full).  Only the evaluator's tables
+// and the small matrices live in LDS, so many instances are resident per CU and no
+// workgroup barrier ever waits on idle waves.  The arithmetic is that of k_relax (the
+// same helpers, the same per-lane order), so both kernels give the same bits and may
+// even continue each other's runs.
+template <int MODE, int R, int KH>
+__global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
+    extern __shared__ __align__(16) char smem[];
+    __shared__ GeomT<float> G;
+    __shared__ LbSlot S;
+    __shared__ unsigned long long bacc[4];
+    const pk_relax_params& p = A.p;
+    const int b = blockIdx.x;
+    if (b >= p.n_launch) return;
+    const int slot = p.order ? p.order[b] : b;
+    const int N = p.ntree[slot];
+    if (N <= 0) return;
+    LbSlot* gS = reinterpret_cast<LbSlot*>(p.state) + slot;
+    const int lane = threadIdx.x;
+    if (lane == 0) S = *gS;
+    load_geom(G, A.geom);
+    __syncthreads();
+    if (S.status == ST_DONE) return;
+
+    const int n = 3 * N + 3, m = p.history;
+    const long long xoff = p.xoff[slot];
+    float* __restrict__ X = p.x + xoff;
+    float* __restrict__ Gg = p.g + xoff;
+    float* __restrict__ Sh = p.s_hist + (long long)m * xoff;
+    float* __restrict__ Yh = p.y_hist + (long long)m * xoff;
+
+    const int Pe = MODE == MODE_LUT ? 0 : G.P;
+    EvalLds<float> L = carve_eval_lds<float>(smem, N, Pe, false, 1, true);
+    L.P = G.P;
+    char* vb = smem + eval_lds_bytes<float>(N, Pe, false, 1, true);
+    float* xs = reinterpret_cast<float*>(vb);      // [64 R] evaluator input
+    float* gs = xs + WAVE * R;                     // [64 R] evaluator output
+    double* SY = reinterpret_cast<double*>(gs + WAVE * R);
+    double* YY = SY + m * m;
+
+    float x[R], g[R], gp[R], d[R];
+    float hs[KH][R], hy[KH][R];
+    const bool run0 = S.status == ST_RUN;
+    const int hl0 = run0 ? S.hist_len : 0, head0 = S.head;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = lane + WAVE * r;
+        const bool in = k < n;
+        x[r] = in ? X[k] : 0.f;
+        g[r] = in ? Gg[k] : 0.f;
+        gp[r] = 0.f;
+        d[r] = 0.f;
+        gs[k] = 0.f;
+#pragma unroll
+        for (int i = 0; i < KH; ++i) {
+            const bool use = in && i < hl0;
+            const long long o = use ? (long long)ring(head0, i, m) * n + k : 0;
+            hs[i][r] = use ? Sh[o] : 0.f;
+            hy[i][r] = use ? Yh[o] : 0.f;
+        }
+    }
+    // rebuild S^T Y and Y^T Y from the stored history
+#pragma unroll
+    for (int j = 0; j < KH; ++j) {
+        if (j < hl0) {
+            double acc[Lb<KH>::KD];
+#pragma unroll
+            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float sv[KH], yv[KH];
+#pragma unroll
+                for (int i = 0; i < KH; ++i) {
+                    sv[i] = hs[i][r];
+                    yv[i] = hy[i][r];
+                }
+                Lb<KH>::dots(acc, sv, yv, hy[j][r], j + 1);
+            }
+            const double rr = wave_sum_many<Lb<KH>::KD>(acc);
+            Lb<KH>::rebuild_col(rr, j, head0, m, SY, YY);
+        }
+    }
+
+    long long st_tp = 0, st_pp = 0;
+    const float skin = 0.5f * G.radius;
+    const float cut_list = (sqrtf(G.cutoff_sq) + skin) * (sqrtf(G.cutoff_sq) + skin);
+    int nb_valid = 0;
+    double vdisp = 0.0;
+
+    if (S.status == ST_FRESH) {
+        const double loss = wave_evaluate<MODE, R>(G, L, A, x, g, xs, gs, N, n, bacc, st_tp, st_pp, 1,
+                                                   cut_list, nb_valid);
+        double gm = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) gm = fmax(gm, fabs((double)g[r]));
+        gm = wave_max(gm);
+        if (lane == 0) {
+            S.loss = loss;
+            S.loss0 = loss;
+            S.H = 1.0;
+            S.n_eval = 1;
+            S.n_iter = 0;
+            S.hist_len = 0;
+            S.head = 0;
+            S.reason = 0;
+            S.status = (gm <= (double)p.tol_grad) ? ST_DONE : ST_RUN;
+            if (S.status == ST_DONE) S.reason = R_GRAD;
+        }
+        __syncthreads();
+    }
+
+    for (int it = 0; it < p.iters; ++it) {
+        if (S.status == ST_DONE) break;
+        // ---- direction, step length, directional-derivative test, step ----
+        const int niter = S.n_iter + 1;
+        const int hl = niter > 1 ? S.hist_len : 0;
+        const int head = S.head;
+        float cs_l = 0.f, cy_l = 0.f;
+        if (hl > 0) {
+            double acc[Lb<KH>::KD];
+#pragma unroll
+            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float sv[KH], yv[KH];
+#pragma unroll
+                for (int i = 0; i < KH; ++i) {
+                    sv[i] = hs[i][r];
+                    yv[i] = hy[i][r];
+                }
+                Lb<KH>::dots(acc, sv, yv, g[r], hl);
+            }
+            const double rr = wave_sum_many<Lb<KH>::KD>(acc);
+            Lb<KH>::solve(rr, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
+        }
+        const float gamf = niter > 1 ? (float)S.H : 1.f;
+        float cs[KH], cy[KH];
+#pragma unroll
+        for (int i = 0; i < KH; ++i) {
+            cs[i] = lane_f(cs_l, i);
+            cy[i] = lane_f(cy_l, i);
+        }
+        double sv2[2] = {0.0, 0.0}, mv[2] = {0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float sv[KH], yv[KH];
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                sv[i] = hs[i][r];
+                yv[i] = hy[i][r];
+            }
+            const float gk = g[r];
+            const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv, hl);
+            d[r] = dk;
+            sv2[0] += (double)gk * (double)dk;
+            sv2[1] += fabs((double)gk);
+            mv[0] = fmax(mv[0], fabs((double)dk));
+            mv[1] = fmax(mv[1], fabs((double)x[r]));
+        }
+        const double rs = wave_sum_many<2>(sv2);
+        const double rm = wave_max_many<2>(mv);
+        const double gtd = lane_d(rs, many_src<2>(0)), gsum = lane_d(rs, many_src<2>(1));
+        const double dmax = lane_d(rm, many_src<2>(0)), xmax = lane_d(rm, many_src<2>(1));
+        double t = niter == 1 ? fmin(1.0, 1.0 / gsum) * (double)p.lr : (double)p.lr;
+        const double pc0 = dmax * fabs(t);
+        if (p.max_step > 0.f && pc0 > (double)p.max_step) t *= (double)p.max_step / pc0;
+        const bool es = gtd > -(double)p.tol_change;
+        if (es) {
+            if (lane == 0) {
+                S.status = ST_DONE;
+                S.reason |= R_DIR;
+            }
+            __syncthreads();
+            break;
+        }
+        const float tf = (float)t;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            gp[r] = g[r];
+            x[r] = x[r] + tf * d[r];
+        }
+        // neighbour-list validity: as k_relax
+        const double step = dmax * fabs(t);
+        vdisp += 1.4142136 * (step * (1.0 + 1e-6) + (xmax + step) * 1.2e-7);
+        int lm = (nb_valid && 2.0 * vdisp < 0.9 * (double)skin) ? 2 : 1;
+        if (lm == 1) vdisp = 0.0;
+        if (lane == 0) S.n_iter = niter;
+        __syncthreads();
+        const double loss_prev = S.loss;
+        double loss = loss_prev;
+        if (niter != p.max_iter)
+            loss = wave_evaluate<MODE, R>(G, L, A, x, g, xs, gs, N, n, bacc, st_tp, st_pp, lm, cut_list,
+                                          nb_valid);
+        // ---- history update and convergence tests ----
+        const int nv = niter > 1 ? hl : 0;
+        double a[Lb<KH>::KA], bb[Lb<KH>::KB];
+#pragma unroll
+        for (int i = 0; i < Lb<KH>::KA; ++i) a[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < Lb<KH>::KB; ++i) bb[i] = 0.0;
+        double mu[2] = {0.0, 0.0};
+        float yn[R], sn[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float sv[KH], yv[KH];
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                sv[i] = hs[i][r];
+                yv[i] = hy[i][r];
+            }
+            const float gk = g[r];
+            yn[r] = gk - gp[r];
+            sn[r] = d[r] * tf;
+            Lb<KH>::upd(a, bb, yn[r], sn[r], sv, yv, nv);
+            mu[0] = fmax(mu[0], fabs((double)gk));
+            mu[1] = fmax(mu[1], fabs((double)sn[r]));
+        }
+        const double rmu = wave_max_many<2>(mu);
+        const double gm = lane_d(rmu, many_src<2>(0)), pc = lane_d(rmu, many_src<2>(1));
+        const double ra = wave_sum_many<Lb<KH>::KA>(a);
+        const double rb = wave_sum_many<Lb<KH>::KB>(bb);
+        const double ys = lane_d(ra, many_src<Lb<KH>::KA>(0));
+        const double yy = lane_d(ra, many_src<Lb<KH>::KA>(1));
+        if (niter > 1 && ys > 1e-10) {
+            const int hl_now = S.hist_len;
+            const int q = hl_now < m ? ring(head, hl_now, m) : head;
+            Lb<KH>::store(ra, rb, ys, yy, hl_now, head, m, q, SY, YY, S);
+            if (hl_now < m) {
+#pragma unroll
+                for (int i = 0; i < KH; ++i) {
+                    if (i == hl_now) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            hs[i][r] = sn[r];
+                            hy[i][r] = yn[r];
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i + 1 < KH; ++i) {
+                    if (i + 1 < m) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            hs[i][r] = hs[i + 1][r];
+                            hy[i][r] = hy[i + 1][r];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < KH; ++i) {
+                    if (i == m - 1) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            hs[i][r] = sn[r];
+                            hy[i][r] = yn[r];
+                        }
+                    }
+                }
+            }
+        }
+        if (lane == 0) {
+            S.n_eval += 1;
+            S.loss = loss;
+            int rr = 0;
+            if (gm <= (double)p.tol_grad) rr |= R_GRAD;
+            if (S.n_eval >= p.max_eval) rr |= R_EVAL;
+            if (pc <= (double)p.tol_change) rr |= R_PARAM;
+            const double lc = fabs(loss - loss_prev);
+            if (lc < (double)p.tol_change) rr |= R_LOSS;
+            if (p.tol_rel > 0.f && niter > m && lc / (fabs(loss_prev) + 1e-12) < (double)p.tol_rel)
+                rr |= R_REL;
+            if (p.stop_on_increase && loss > S.loss0) rr |= R_INCREASE;
+            if (niter >= p.max_iter) rr |= R_ITER;
+            if (rr) {
+                S.status = ST_DONE;
+                S.reason |= rr;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- write back ----
+    const int hl_f = S.hist_len, head_f = S.head;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int k = lane + WAVE * r;
+        if (k < n) {
+            X[k] = x[r];
+            Gg[k] = g[r];
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                if (i < hl_f) {
+                    const long long o = (long long)ring(head_f, i, m) * n + k;
+                    Sh[o] = hs[i][r];
+                    Yh[o] = hy[i][r];
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        *gS = S;
+        if (p.status_out) p.status_out[slot] = S.status == ST_DONE ? 1 : 0;
+        if (p.loss_out) p.loss_out[slot] = (float)S.loss;
+    }
+    if (p.stats && lane == 0) {
+        atomicAdd(&p.stats[0], (unsigned long long)st_tp);
+        atomicAdd(&p.stats[1], (unsigned long long)st_pp);
+    }
+}
+
+// Dynamic LDS of k_relax_w: evaluator (one wave), evaluator input/output vectors, small
+// matrices.
+size_t relax_w_lds_bytes(int mode, int n_max, int P, int m, int R) {
+    const int Pe = mode == MODE_LUT ? 0 : P;
+    size_t b = eval_lds_bytes<float>(n_max, Pe, false, 1, true);
+    b += sizeof(float) * 2 * WAVE * (size_t)R;
+    b += sizeof(double) * 2 * (size_t)m * m;
+    return align_up(b, 16);
+}
+
 template <int MODE, int BLOCK, bool HL>
 static int launch_relax(const Geometry* g, const RelaxArgs& A, size_t lds, hipStream_t st) {
-    auto kern = k_relax<MODE, BLOCK, HL>;
-    PK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(A.p.n_launch), dim3(BLOCK), lds, st, A);
+    const void* kern = A.p.history <= 8 ? reinterpret_cast<const void*>(k_relax<MODE, BLOCK, HL, 8>)
+                                        : reinterpret_cast<const void*>(k_relax<MODE, BLOCK, HL, 16>);
+    PK_CHECK_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (A.p.history <= 8)
+        hipLaunchKernelGGL((k_relax<MODE, BLOCK, HL, 8>), dim3(A.p.n_launch), dim3(BLOCK), lds, st, A);
+    else
+        hipLaunchKernelGGL((k_relax<MODE, BLOCK, HL, 16>), dim3(A.p.n_launch), dim3(BLOCK), lds, st, A);
     PK_CHECK_HIP(hipGetLastError());
     return 0;
 }
@@ -487,6 +1020,53 @@ static int launch_relax(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
     return launch_relax<MODE, BLOCK, false>(g, A, lds_g, st);
 }
 
+// One-wave kernel: rows of the state vector per lane, and the largest supported count.
+constexpr int W_RMAX = 10;
+constexpr int W_KH = 8;   // history rows held in registers
+inline int wave_rows(int n_tree_max) { return (3 * n_tree_max + 3 + WAVE - 1) / WAVE; }
+
+template <int MODE, int R, int KH>
+static int launch_relax_w_rk(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
+    int dev = 0;
+    PK_CHECK_HIP(hipGetDevice(&dev));
+    int lim = 0;
+    PK_CHECK_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t lds = relax_w_lds_bytes(MODE, A.p.n_tree_max, g->hf.P, A.p.history, R);
+    const size_t stat = sizeof(GeomT<float>) + sizeof(LbSlot) + 64;
+    if (lds + stat > (size_t)lim) {
+        set_error("pk_relax_step: instance of " + std::to_string(A.p.n_tree_max) + " trees needs " +
+                  std::to_string(lds + stat) + " B of LDS, device allows " + std::to_string(lim));
+        return PK_ERANGE;
+    }
+    auto kern = k_relax_w<MODE, R, KH>;
+    PK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(A.p.n_launch), dim3(WAVE), lds, st, A);
+    PK_CHECK_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int MODE, int R>
+static int launch_relax_w_r(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
+    return launch_relax_w_rk<MODE, R, W_KH>(g, A, st);
+}
+
+template <int MODE>
+static int launch_relax_w(const Geometry* g, const RelaxArgs& A, int rows, hipStream_t st) {
+    switch (rows) {
+        case 1: return launch_relax_w_r<MODE, 1>(g, A, st);
+        case 2: return launch_relax_w_r<MODE, 2>(g, A, st);
+        case 3: return launch_relax_w_r<MODE, 3>(g, A, st);
+        case 4: return launch_relax_w_r<MODE, 4>(g, A, st);
+        case 5: return launch_relax_w_r<MODE, 5>(g, A, st);
+        case 6: return launch_relax_w_r<MODE, 6>(g, A, st);
+        case 7: return launch_relax_w_r<MODE, 7>(g, A, st);
+        case 8: return launch_relax_w_r<MODE, 8>(g, A, st);
+        case 9: return launch_relax_w_r<MODE, 9>(g, A, st);
+        default: return launch_relax_w_r<MODE, 10>(g, A, st);
+    }
+}
+
 LutDev make_lut_dev(const void* lut_handle);
 
 }  // namespace pk
@@ -500,6 +1080,7 @@ int pk_relax_slot_bytes(void) { return (int)sizeof(LbSlot); }
 int pk_relax_lds_bytes(pk_geometry_t h, int mode, int n_tree_max, int history, int hist_lds, int block) {
     const Geometry* g = reinterpret_cast<const Geometry*>(h);
     if (!g || n_tree_max < 1 || history < 1) return PK_EINVAL;
+    if (block == 1) return (int)relax_w_lds_bytes(mode, n_tree_max, g->hf.P, history, wave_rows(n_tree_max));
     if (block != 64 && block != 128 && block != 256) return PK_EINVAL;
     return (int)relax_lds_bytes(mode, n_tree_max, g->hf.P, history, hist_lds != 0, block);
 }
@@ -524,13 +1105,27 @@ int pk_relax_step(pk_geometry_t h, const pk_relax_params* p, void* stream) {
     std::memset(&A.lut, 0, sizeof(A.lut));
     if (p->mode == PK_MODE_LUT) A.lut = make_lut_dev(p->lut);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int blk = p->block ? p->block : 256;
+    const int rows = wave_rows(p->n_tree_max);
+    int blk = p->block;
+    const bool wave_ok = rows <= W_RMAX && p->history <= W_KH;
+    if (blk == 0) blk = wave_ok ? 1 : 256;
+    if (blk == 1) {
+        if (!wave_ok) {
+            set_error("pk_relax_step: block 1 (one wave per instance) supports up to " +
+                      std::to_string((W_RMAX * WAVE - 3) / 3) + " trees and history " +
+                      std::to_string(W_KH));
+            return PK_ERANGE;
+        }
+        if (p->mode == PK_MODE_AREA) return launch_relax_w<MODE_AREA>(g, A, rows, st);
+        if (p->mode == PK_MODE_SEP) return launch_relax_w<MODE_SEP>(g, A, rows, st);
+        return launch_relax_w<MODE_LUT>(g, A, rows, st);
+    }
 #define PK_RL(M)                                                          \
     switch (blk) {                                                        \
         case 64: return launch_relax<M, 64>(g, A, st);                    \
         case 128: return launch_relax<M, 128>(g, A, st);                  \
         case 256: return launch_relax<M, 256>(g, A, st);                  \
-        default: set_error("pk_relax_step: block must be 64, 128 or 256"); \
+        default: set_error("pk_relax_step: block must be 0, 1, 64, 128 or 256"); \
             return PK_EINVAL;                                             \
     }
     if (p->mode == PK_MODE_AREA) { PK_RL(MODE_AREA) }

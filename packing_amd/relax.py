@@ -44,7 +44,7 @@ class RelaxOptions:
     tol_change: float = 0.0
     tol_rel: float = 0.0
     stop_on_increase: bool = False
-    block: int = 0          # 0: chosen by the tile table (autotune.py), per bucket
+    block: int = 0          # 0: tile table (autotune.py), else the library's choice
     buckets: int = 4        # size buckets launched concurrently (1: one launch)
     hist_lds: int = -1      # L-BFGS history in LDS: -1 when it fits, 0 never, 1 always
     lut: object = None      # pack_cuda_lut.LookupTable for mode "lut"

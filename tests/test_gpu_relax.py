@@ -56,7 +56,7 @@ def test_invariances():
     ref = batch(geom, xs, hs, **kw)
     ref.run()
     R = [ref.xyt(i).cpu().numpy() for i in range(len(sizes))]
-    for blk in (64, 128, 256):
+    for blk in (1, 64, 128, 256):
         b = batch(geom, xs, hs, block=blk, **kw)
         b.run(iters_per_launch=7)
         for i in range(len(sizes)):
@@ -118,3 +118,52 @@ def test_lut_mode_relaxes():
     b.run()
     torch.cuda.synchronize()
     assert (b.loss.cpu().numpy() < l0).all()
+
+
+@pytest.mark.parametrize("mode", ["lut", "area"])
+def test_wave_kernel_matches_lds_kernel(mode):
+    """The one-wave kernel (state and history in registers) and the LDS kernel give the
+    same bits, also when a run switches kernels between launches (the stored history is
+    read back by either), near the one-wave size limit and with a short history."""
+    from packing_amd.context import set_geometry
+    from packing_amd.pack_cuda_lut import LookupTable
+    geom = GEOMS["random"]()
+    set_geometry(geom)
+    kw = dict(mode=mode, max_iter=40, history=8, max_step=0.1)
+    if mode == "lut":
+        kw["lut"] = LookupTable.build_exact(96, 96, 64, full_x=True)
+    sizes = [212, 150, 21, 22, 43, 5] if mode == "lut" else [60, 21, 22, 43, 5]
+    xs, hs = ragged(7, sizes)
+    for hist in (8, 3):
+        kw["history"] = hist
+        ref = batch(geom, xs, hs, block=128, **kw)
+        ref.run()
+        one = batch(geom, xs, hs, block=1, **kw)
+        one.run(iters_per_launch=13)
+        mix = batch(geom, xs, hs, block=1, **kw)
+        for k, blk in enumerate((1, 256, 1, 64)):
+            mix.opts.block = blk
+            mix.launch(10)
+        for i in range(len(sizes)):
+            r = ref.xyt(i).cpu().numpy()
+            assert np.array_equal(one.xyt(i).cpu().numpy(), r), ("one", hist, i)
+            assert np.array_equal(mix.xyt(i).cpu().numpy(), r), ("mix", hist, i)
+        assert np.array_equal(one.slot_records(), ref.slot_records())
+
+
+def test_wave_kernel_limits():
+    """Block 1 takes up to 212 trees and history 8; beyond that the library's own choice
+    falls back to the LDS kernel and an explicit block 1 is rejected."""
+    geom = GEOMS["tree"]()
+    xs, hs = ragged(2, [213])
+    b = batch(geom, xs, hs, mode="area", max_iter=3, history=8, max_step=0.1, block=1)
+    with pytest.raises(ValueError):
+        b.run()
+    a = batch(geom, xs, hs, mode="area", max_iter=3, history=8, max_step=0.1)
+    a.run()
+    c = batch(geom, xs, hs, mode="area", max_iter=3, history=8, max_step=0.1, block=256)
+    c.run()
+    assert np.array_equal(a.xyt(0).cpu().numpy(), c.xyt(0).cpu().numpy())
+    h = batch(geom, *ragged(3, [30]), mode="area", max_iter=3, history=9, max_step=0.1, block=1)
+    with pytest.raises(ValueError):
+        h.run()
