@@ -199,6 +199,43 @@ __device__ inline double wave_sum_many(double (&v)[K]) {
     for (int s = LK; s < 6; ++s) r += shfl_xor_d(r, 32 >> s);
     return r;
 }
+// The same for float partial sums, added in double.
+template <int K>
+__device__ inline double wave_sum_many(float (&v)[K]) {
+    double dv[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) dv[j] = (double)v[j];
+    return wave_sum_many<K>(dv);
+}
+// The same butterfly in float (one exchange per value and stage).
+__device__ inline void blend_pair_f(float lo, float hi, bool up, float& send, float& keep) {
+    const unsigned m = up ? ~0u : 0u;
+    const unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
+    send = __uint_as_float((a & m) | (b & ~m));
+    keep = __uint_as_float((b & m) | (a & ~m));
+}
+template <int K>
+__device__ inline float wave_sum_many_f(float (&v)[K]) {
+    constexpr int LK = ilog2c(K);
+    static_assert((1 << LK) == K && K >= 2 && K <= 64, "K must be a power of two in [2, 64]");
+    const int lane = __lane_id();
+#pragma unroll
+    for (int s = 0; s < LK; ++s) {
+        const int bit = 32 >> s;
+        const int h = K >> (s + 1);
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            float send, keep;
+            blend_pair_f(v[j], v[h + j], up, send, keep);
+            v[j] = keep + __shfl_xor(send, bit);
+        }
+    }
+    float r = v[0];
+#pragma unroll
+    for (int s = LK; s < 6; ++s) r += __shfl_xor(r, 32 >> s);
+    return r;
+}
 // The same layout with max instead of sum (exact in any order).
 template <int K>
 __device__ inline double wave_max_many(double (&v)[K]) {

@@ -37,34 +37,70 @@ struct LutDev {
     int mirror;    // table covers x >= 0; x < 0 looked up as (-x, -theta)
 };
 
+// One table lookup split in two so that the loads of several lookups can be in flight
+// together: lut_fetch computes the cell and issues the loads (the two theta-neighbour
+// corners of each (x, y) node come in one 8-byte load), lut_combine interpolates.
+struct LutTap {
+    float v000, v001, v010, v011, v100, v101, v110, v111;
+    float fx, fy, ft;
+    bool flip, in;
+};
+
+__device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, LutTap& t) {
+    t.in = !(x < T_.x_lo || x > T_.x_max || y < T_.y_min || y > T_.y_max || th < T_.t_min || th > T_.t_max);
+    t.flip = T_.mirror && x < 0.f;
+    const float xt = t.flip ? -x : x, tt = t.flip ? -th : th;
+    const float gx = (xt - T_.x_min) * T_.sx, gy = (y - T_.y_min) * T_.sy, gt = (tt - T_.t_min) * T_.st;
+    int ix0 = (int)floorf(gx), iy0 = (int)floorf(gy), it0 = (int)floorf(gt);
+    if (!t.in) ix0 = iy0 = it0 = 0;   // any cell of the table; the values are not used
+    const int ix1 = min(ix0 + 1, T_.nx - 1), iy1 = min(iy0 + 1, T_.ny - 1);
+    t.fx = gx - ix0;
+    t.fy = gy - iy0;
+    t.ft = gt - it0;
+    // theta pair (it0, min(it0 + 1, nt - 1)) read as nodes (bt, bt + 1), bt <= nt - 2
+    const int bt = min(it0, T_.nt - 2);
+    const bool top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
+    const float* __restrict__ v = T_.vals;
+    const size_t r00 = ((size_t)ix0 * T_.ny + iy0) * T_.nt + bt, r01 = ((size_t)ix0 * T_.ny + iy1) * T_.nt + bt;
+    const size_t r10 = ((size_t)ix1 * T_.ny + iy0) * T_.nt + bt, r11 = ((size_t)ix1 * T_.ny + iy1) * T_.nt + bt;
+    float2 p00, p01, p10, p11;
+    __builtin_memcpy(&p00, v + r00, 8);
+    __builtin_memcpy(&p01, v + r01, 8);
+    __builtin_memcpy(&p10, v + r10, 8);
+    __builtin_memcpy(&p11, v + r11, 8);
+    t.v000 = top ? p00.y : p00.x;
+    t.v001 = p00.y;
+    t.v010 = top ? p01.y : p01.x;
+    t.v011 = p01.y;
+    t.v100 = top ? p10.y : p10.x;
+    t.v101 = p10.y;
+    t.v110 = top ? p11.y : p11.x;
+    t.v111 = p11.y;
+}
+
 // Trilinear value and analytic gradient w.r.t. (x, y, theta) of the table
 // (core/pack_cuda_lut.py:298-415): outside the grid range 0; clamped upper corner.
-__device__ inline float lut_lookup(const LutDev& T_, float x, float y, float th, float (&g)[3]) {
+__device__ inline float lut_combine(const LutDev& T_, const LutTap& t, float (&g)[3]) {
     g[0] = g[1] = g[2] = 0.f;
-    if (x < T_.x_lo || x > T_.x_max || y < T_.y_min || y > T_.y_max || th < T_.t_min || th > T_.t_max)
-        return 0.f;
-    const bool flip = T_.mirror && x < 0.f;
-    const float xt = flip ? -x : x, tt = flip ? -th : th;
-    const float gx = (xt - T_.x_min) * T_.sx, gy = (y - T_.y_min) * T_.sy, gt = (tt - T_.t_min) * T_.st;
-    const int ix0 = (int)floorf(gx), iy0 = (int)floorf(gy), it0 = (int)floorf(gt);
-    const int ix1 = min(ix0 + 1, T_.nx - 1), iy1 = min(iy0 + 1, T_.ny - 1), it1 = min(it0 + 1, T_.nt - 1);
-    const float fx = gx - ix0, fy = gy - iy0, ft = gt - it0;
-    const float* __restrict__ v = T_.vals;
-    const size_t r00 = ((size_t)ix0 * T_.ny + iy0) * T_.nt, r01 = ((size_t)ix0 * T_.ny + iy1) * T_.nt;
-    const size_t r10 = ((size_t)ix1 * T_.ny + iy0) * T_.nt, r11 = ((size_t)ix1 * T_.ny + iy1) * T_.nt;
-    const float v000 = v[r00 + it0], v001 = v[r00 + it1], v010 = v[r01 + it0], v011 = v[r01 + it1];
-    const float v100 = v[r10 + it0], v101 = v[r10 + it1], v110 = v[r11 + it0], v111 = v[r11 + it1];
-    const float v00 = v000 * (1.f - ft) + v001 * ft, v01 = v010 * (1.f - ft) + v011 * ft;
-    const float v10 = v100 * (1.f - ft) + v101 * ft, v11 = v110 * (1.f - ft) + v111 * ft;
+    if (!t.in) return 0.f;
+    const float fx = t.fx, fy = t.fy, ft = t.ft;
+    const float v00 = t.v000 * (1.f - ft) + t.v001 * ft, v01 = t.v010 * (1.f - ft) + t.v011 * ft;
+    const float v10 = t.v100 * (1.f - ft) + t.v101 * ft, v11 = t.v110 * (1.f - ft) + t.v111 * ft;
     const float v0 = v00 * (1.f - fy) + v01 * fy, v1 = v10 * (1.f - fy) + v11 * fy;
     const float val = v0 * (1.f - fx) + v1 * fx;
-    const float d0 = (v001 - v000) * (1.f - fy) + (v011 - v010) * fy;
-    const float d1 = (v101 - v100) * (1.f - fy) + (v111 - v110) * fy;
+    const float d0 = (t.v001 - t.v000) * (1.f - fy) + (t.v011 - t.v010) * fy;
+    const float d1 = (t.v101 - t.v100) * (1.f - fy) + (t.v111 - t.v110) * fy;
     g[0] = (v1 - v0) * T_.sx;
     g[1] = ((1.f - fx) * (v01 - v00) + fx * (v11 - v10)) * T_.sy;
     g[2] = (d0 * (1.f - fx) + d1 * fx) * T_.st;
-    if (flip) { g[0] = -g[0]; g[2] = -g[2]; }
+    if (t.flip) { g[0] = -g[0]; g[2] = -g[2]; }
     return val;
+}
+
+__device__ inline float lut_lookup(const LutDev& T_, float x, float y, float th, float (&g)[3]) {
+    LutTap t;
+    lut_fetch(T_, x, y, th, t);
+    return lut_combine(T_, t, g);
 }
 
 // wrap to [-pi, pi] (core/pack_cuda_lut.py:289-293)
@@ -527,13 +563,25 @@ struct Evaluator {
 
     // Table lookup for tree pair (i, j) (core/pack_cuda_lut.py:420-547): relative pose
     // of j in i's frame, value (optionally max(0,v)^2) and gradient to both trees.
-    __device__ inline void lut_pair(int i, int j) {
+    struct LutPair {
+        int i, j;
+        float cr, sr, dxl, dyl;
+        LutTap tap;
+    };
+    __device__ inline void lut_begin(int i, int j, LutPair& q) const {
+        q.i = i;
+        q.j = j;
         const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-        const float cr = L.c[i], sr = L.s[i];
-        const float dxl = cr * dx + sr * dy, dyl = -sr * dx + cr * dy;
+        q.cr = L.c[i];
+        q.sr = L.s[i];
+        q.dxl = q.cr * dx + q.sr * dy;
+        q.dyl = -q.sr * dx + q.cr * dy;
         const float dth = wrap_pi(L.t[j] - L.t[i]);
+        lut_fetch(lut, q.dxl, q.dyl, dth, q.tap);
+    }
+    __device__ inline void lut_end(const LutPair& q) {
         float gl[3];
-        float v = lut_lookup(lut, dxl, dyl, dth, gl);
+        float v = lut_combine(lut, q.tap, gl);
         if (lut.quad) {
             const float k = v > 0.f ? 2.f * v : 0.f;
             gl[0] *= k; gl[1] *= k; gl[2] *= k;
@@ -541,13 +589,33 @@ struct Evaluator {
         }
         lane_cost += AccT::local(v);
         if (GRAD) {
+            const int i = q.i, j = q.j;
+            const float cr = q.cr, sr = q.sr;
             AccT::add(&L.g[3 * i + 0], -gl[0] * cr + gl[1] * sr);
             AccT::add(&L.g[3 * i + 1], -gl[0] * sr - gl[1] * cr);
-            AccT::add(&L.g[3 * i + 2], gl[0] * dyl - gl[1] * dxl - gl[2]);
+            AccT::add(&L.g[3 * i + 2], gl[0] * q.dyl - gl[1] * q.dxl - gl[2]);
             AccT::add(&L.g[3 * j + 0], gl[0] * cr - gl[1] * sr);
             AccT::add(&L.g[3 * j + 1], gl[0] * sr + gl[1] * cr);
             AccT::add(&L.g[3 * j + 2], gl[2]);
         }
+    }
+    // Lookups for cnt <= 128 queue entries, two per lane, with the table loads of both in
+    // flight before either is interpolated.
+    __device__ inline void lut_pairs(const uint32_t* q, int cnt) {
+        const int lane = threadIdx.x % WAVE;
+        const bool a = lane < cnt, b = lane + WAVE < cnt;
+        LutPair pa, pb;
+        if (a) {
+            const uint32_t e = q[lane];
+            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pa);
+        }
+        if (b) {
+            const uint32_t e = q[lane + WAVE];
+            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pb);
+        }
+        if (a) lut_end(pa);
+        if (b) lut_end(pb);
+        n_pp += cnt;
     }
 
     // ---------------------------------------------------------------- grid path
@@ -611,16 +679,16 @@ struct Evaluator {
             tn += __popcll(m);
             n_tp += __popcll(m);
             wave_sync();
-            if (tn >= WAVE) {
-                if (MODE == MODE_LUT) {
-                    const uint32_t e = tq[tn - WAVE + lane];
-                    lut_pair(e & 0xfff, (e >> 12) & 0xfff);
-                    tn -= WAVE;
-                    n_pp += WAVE;
-                } else {
-                    expand(tq, tn, pq, pn);
-                    tn = 0;
+            if (MODE == MODE_LUT) {
+                // the table queue uses both halves of the wave's queue space (2 QCAP entries)
+                if (tn >= 2 * WAVE) {
+                    lut_pairs(tq + tn - 2 * WAVE, 2 * WAVE);
+                    tn -= 2 * WAVE;
+                    wave_sync();
                 }
+            } else if (tn >= WAVE) {
+                expand(tq, tn, pq, pn);
+                tn = 0;
                 wave_sync();
             }
         }
@@ -705,11 +773,7 @@ struct Evaluator {
             }
         }
         if (MODE == MODE_LUT) {
-            if (lane < tn) {
-                const uint32_t e = tq[lane];
-                lut_pair(e & 0xfff, (e >> 12) & 0xfff);
-            }
-            n_pp += tn;
+            lut_pairs(tq, tn);
         } else {
             expand(tq, tn, pq, pn);
             if (lane < pn) clip_entry(pq[lane]);

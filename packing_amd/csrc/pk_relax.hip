@@ -162,8 +162,9 @@ struct Hist {
 
 // ---------------------------------------------------------------------------------
 // L-BFGS arithmetic shared by both relaxation kernels.  Vectors are walked element by
-// element, k = lane + 64 r with r increasing; per-lane partial sums are double and every
-// wave-wide sum goes through wave_sum_many, whose result does not depend on the list it
+// element, k = lane + 64 r with r increasing; per-lane partial sums are float fused
+// multiply-adds (at most a few terms per lane), and every wave-wide sum adds the 64
+// partials in double through wave_sum_many, whose result does not depend on the list it
 // sits in.  Both kernels therefore produce the same bits, wherever the vectors live.
 // History rows are passed in chronological order (oldest first); rows at or beyond the
 // used count are ignored.
@@ -180,20 +181,22 @@ __host__ __device__ inline int ring(int head, int i, int m) {
 
 template <int KH>
 struct Lb {
-    static constexpr int KD = 2 * KH;             // dots: i = s_i.v, KH + i = y_i.v
-    static constexpr int KA = pow2_ceil(2 + KH);  // update A: 0 = y.s, 1 = y.y, 2 + i = s_i.y
-    static constexpr int KB = KH;                 // update B: i = y_i.y
+    static constexpr int KD = 2 * KH;   // dots: i = s_i.v, KH + i = y_i.v
 
-    __device__ static inline void dots(double (&acc)[KD], const float (&sv)[KH], const float (&yv)[KH],
+    __device__ static inline void dots(float (&acc)[KD], const float (&sv)[KH], const float (&yv)[KH],
                                        float v, int nv) {
-        const double vd = (double)v;
 #pragma unroll
         for (int i = 0; i < KH; ++i) {
             if (i < nv) {
-                acc[i] += (double)sv[i] * vd;
-                acc[KH + i] += (double)yv[i] * vd;
+                acc[i] = fmaf(sv[i], v, acc[i]);
+                acc[KH + i] = fmaf(yv[i], v, acc[KH + i]);
             }
         }
+    }
+    // y.s and y.y of a pair (per-lane partials; summed over the wave in double)
+    __device__ static inline void pair(float (&a)[2], float yk, float sk) {
+        a[0] = fmaf(yk, sk, a[0]);
+        a[1] = fmaf(yk, yk, a[1]);
     }
     // d_k = -gamma g_k + sum_i (cs_i s_i,k + cy_i y_i,k), in float, in this order
     __device__ static inline float dir(float gamf, float gk, const float (&cs)[KH], const float (&cy)[KH],
@@ -208,19 +211,6 @@ struct Lb {
         }
         return dk;
     }
-    __device__ static inline void upd(double (&a)[KA], double (&b)[KB], float yk, float sk,
-                                      const float (&sv)[KH], const float (&yv)[KH], int nv) {
-        const double yd = (double)yk;
-        a[0] += yd * (double)sk;
-        a[1] += yd * yd;
-#pragma unroll
-        for (int i = 0; i < KH; ++i) {
-            if (i < nv) {
-                a[2 + i] += (double)sv[i] * yd;
-                b[i] += (double)yv[i] * yd;
-            }
-        }
-    }
 
     // Direction coefficients of the compact L-BFGS form (Byrd, Nocedal and Schnabel 1994,
     // eq. 3.1): with a = S^T g, b = Y^T g (r: the dots butterfly), R = upper part of S^T Y,
@@ -230,43 +220,53 @@ struct Lb {
     // The small matrices are indexed by ring slot, (head + i) mod m for chronological i.  The
     // hl x hl triangular solves run one row per lane.  Returns, in lane i < hl, the
     // coefficients of d = -gamma g + sum_i cs_i s_i + cy_i y_i.
-    __device__ static inline void solve(double r, int hl, int head, int m, double gam, const double* SY,
+    __device__ static inline void solve(float r, int hl, int head, int m, double gam, const double* SY,
                                         const double* YY, const double* ro, float& cs_l, float& cy_l) {
         const int lane = __lane_id();
-        const double a_l = shfl_d(r, many_src<KD>(lane & (KH - 1)));
-        const double b_l = shfl_d(r, many_src<KD>(KH + (lane & (KH - 1))));
+        const int li = lane & (KH - 1);
+        const double a_l = (double)__shfl(r, many_src<KD>(li));
+        const double b_l = (double)__shfl(r, many_src<KD>(KH + li));
         const bool row = lane < hl;
         const int pl = ring(head, lane < m ? lane : 0, m);
         double acc = a_l, u_l = 0.0, zacc = 0.0;
-        for (int j = hl - 1; j >= 0; --j) {
-            const int pj = ring(head, j, m);
-            const double uj = lane_d(acc, j) * ro[pj];
-            if (lane == j) u_l = uj;
-            if (lane < j) acc -= SY[pl * m + pj] * uj;
-            if (row) zacc += YY[pl * m + pj] * uj;
+#pragma unroll
+        for (int j = KH - 1; j >= 0; --j) {
+            const int pj = ring(head, j < m ? j : 0, m);
+            const double sy = SY[pl * m + pj], yy = YY[pl * m + pj], rj = ro[pj];
+            if (j < hl) {
+                const double uj = lane_d(acc, j) * rj;
+                if (lane == j) u_l = uj;
+                if (lane < j) acc -= sy * uj;
+                if (row) zacc += yy * uj;
+            }
         }
         const double z_l = row ? SY[pl * m + pl] * u_l + gam * zacc - gam * b_l : 0.0;
         double accw = z_l, w_l = 0.0;
-        for (int j = 0; j < hl; ++j) {
-            const int pj = ring(head, j, m);
-            const double wj = lane_d(accw, j) * ro[pj];
-            if (lane == j) w_l = wj;
-            if (row && lane > j) accw -= SY[pj * m + pl] * wj;
+#pragma unroll
+        for (int j = 0; j < KH; ++j) {
+            const int pj = ring(head, j < m ? j : 0, m);
+            const double sy = SY[pj * m + pl], rj = ro[pj];
+            if (j < hl) {
+                const double wj = lane_d(accw, j) * rj;
+                if (lane == j) w_l = wj;
+                if (row && lane > j) accw -= sy * wj;
+            }
         }
         cs_l = row ? (float)(-w_l) : 0.f;
         cy_l = row ? (float)(gam * u_l) : 0.f;
     }
 
-    // Accept the pair into ring slot q: new column of S^T Y and row of Y^T Y (ra, rb: the
-    // update butterflies over the hl rows in use; a dropped oldest row is harmless),
-    // H_diag, ro and the ring counters.
-    __device__ static inline void store(double ra, double rb, double ys, double yy, int hl, int head, int m,
-                                        int q, double* SY, double* YY, LbSlot& S) {
+    // Accept the pair into ring slot q: new column of S^T Y and row of Y^T Y from r (the
+    // dots butterfly of y_new over the hl rows in use; a dropped oldest row is harmless),
+    // the diagonal from the pair's own sums, H_diag, ro and the ring counters.
+    __device__ static inline void store(float r, double ys, double yy, int hl, int head, int m, int q,
+                                        double* SY, double* YY, LbSlot& S) {
         const int lane = __lane_id();
-        const double syc = shfl_d(ra, many_src<KA>(2 + (lane & (KH - 1))));
-        const double yyr = shfl_d(rb, many_src<KB>(lane & (KH - 1)));
+        const int li = lane & (KH - 1);
+        const double syc = (double)__shfl(r, many_src<KD>(li));
+        const double yyr = (double)__shfl(r, many_src<KD>(KH + li));
         if (lane < hl) {
-            const int pi = ring(head, lane < m ? lane : 0, m);
+            const int pi = ring(head, lane, m);
             if (pi != q) {
                 SY[pi * m + q] = syc;
                 YY[pi * m + q] = yyr;
@@ -283,16 +283,24 @@ struct Lb {
         }
     }
 
-    // S^T Y (upper) and Y^T Y for column j (chronological) from the dots butterfly of y_j.
-    __device__ static inline void rebuild_col(double r, int j, int head, int m, double* SY, double* YY) {
+    // Column j (chronological) of S^T Y (upper) and Y^T Y from the dots butterfly r of y_j
+    // and the pair sums of (s_j, y_j), as store() wrote them.
+    __device__ static inline void rebuild_col(float r, double dsy, double dyy, int j, int head, int m,
+                                              double* SY, double* YY) {
         const int lane = __lane_id();
-        const double sy = shfl_d(r, many_src<KD>(lane & (KH - 1)));
-        const double yy = shfl_d(r, many_src<KD>(KH + (lane & (KH - 1))));
-        if (lane <= j) {
-            const int pi = ring(head, lane < m ? lane : 0, m), pj = ring(head, j, m);
+        const int li = lane & (KH - 1);
+        const double sy = (double)__shfl(r, many_src<KD>(li));
+        const double yy = (double)__shfl(r, many_src<KD>(KH + li));
+        const int pj = ring(head, j, m);
+        if (lane < j) {
+            const int pi = ring(head, lane, m);
             SY[pi * m + pj] = sy;
             YY[pi * m + pj] = yy;
             YY[pj * m + pi] = yy;
+        }
+        if (lane == 0) {
+            SY[pj * m + pj] = dsy;
+            YY[pj * m + pj] = dyy;
         }
     }
 };
@@ -315,10 +323,10 @@ __device__ inline void hist_offsets(const Hist<HL>& H, int head, int m, int (&of
 
 // Dots of v with the first nv history rows (LDS / global history).  Wave 0 only.
 template <int KH, bool HL, typename VF>
-__device__ inline double lb_dots(int np, int nv, int head, int m, const Hist<HL>& H, VF v) {
-    double acc[Lb<KH>::KD];
+__device__ inline float lb_dots(int np, int nv, int head, int m, const Hist<HL>& H, VF v) {
+    float acc[Lb<KH>::KD];
 #pragma unroll
-    for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.0;
+    for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
     int off[KH];
     hist_offsets<KH, HL>(H, head, m, off);
     for (int k = threadIdx.x; k < np; k += WAVE) {
@@ -326,18 +334,21 @@ __device__ inline double lb_dots(int np, int nv, int head, int m, const Hist<HL>
         hist_elem<KH, HL>(H, off, k, nv, sv, yv);
         Lb<KH>::dots(acc, sv, yv, v(k), nv);
     }
-    return wave_sum_many<Lb<KH>::KD>(acc);
+    return wave_sum_many_f<Lb<KH>::KD>(acc);
 }
 
-// Rebuild S^T Y and Y^T Y from the stored history rows (same sums as the incremental
+// Rebuild S^T Y and Y^T Y from the stored history rows (the same sums as the incremental
 // updates, so a call that starts from stored history continues bit for bit).  Wave 0 only.
 template <int KH, bool HL>
 __device__ inline void lb_rebuild(int np, int hl, int head, int m, const Hist<HL>& H, double* SY,
                                   double* YY) {
     for (int j = 0; j < hl; ++j) {
         const int oj = ring(head, j, m) * H.stride;
-        const double r = lb_dots<KH, HL>(np, j + 1, head, m, H, [&](int k) { return H.Y(oj, k); });
-        Lb<KH>::rebuild_col(r, j, head, m, SY, YY);
+        const float r = lb_dots<KH, HL>(np, j, head, m, H, [&](int k) { return H.Y(oj, k); });
+        float pa[2] = {0.f, 0.f};
+        for (int k = threadIdx.x; k < np; k += WAVE) Lb<KH>::pair(pa, H.Y(oj, k), H.S(oj, k));
+        const double rp = wave_sum_many<2>(pa);
+        Lb<KH>::rebuild_col(r, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), j, head, m, SY, YY);
     }
 }
 
@@ -373,37 +384,6 @@ __device__ inline void lb_direction(int np, int hl, int head, int m, float gamf,
     gsum = lane_d(rs, many_src<2>(1));
     dmax = lane_d(rm, many_src<2>(0));
     xmax = lane_d(rm, many_src<2>(1));
-}
-
-// Update pass after an evaluation: with y = g - g_prev and s = t d, the sums y.s, y.y,
-// s_i.y (ra) and y_i.y (rb) for the nv history rows in use, and max |g|, max |s|.  Wave 0.
-template <int KH, bool HL>
-__device__ inline void lb_update_dots(int np, int nv, int head, int m, float tf, const Hist<HL>& H,
-                                      const float* g, const float* gp, const float* d, double& ra,
-                                      double& rb, double& gmax, double& smax) {
-    double a[Lb<KH>::KA], bb[Lb<KH>::KB];
-#pragma unroll
-    for (int i = 0; i < Lb<KH>::KA; ++i) a[i] = 0.0;
-#pragma unroll
-    for (int i = 0; i < Lb<KH>::KB; ++i) bb[i] = 0.0;
-    int off[KH];
-    hist_offsets<KH, HL>(H, head, m, off);
-    double mv[2] = {0.0, 0.0};
-    for (int k = threadIdx.x; k < np; k += WAVE) {
-        float sv[KH], yv[KH];
-        hist_elem<KH, HL>(H, off, k, nv, sv, yv);
-        const float gk = g[k];
-        const float yk = gk - gp[k];
-        const float sk = d[k] * tf;
-        Lb<KH>::upd(a, bb, yk, sk, sv, yv, nv);
-        mv[0] = fmax(mv[0], fabs((double)gk));
-        mv[1] = fmax(mv[1], fabs((double)sk));
-    }
-    const double rm = wave_max_many<2>(mv);
-    gmax = lane_d(rm, many_src<2>(0));
-    smax = lane_d(rm, many_src<2>(1));
-    ra = wave_sum_many<Lb<KH>::KA>(a);
-    rb = wave_sum_many<Lb<KH>::KB>(bb);
 }
 
 // HL: the history s_hist / y_hist is copied into LDS at the start of the call and back at
@@ -522,7 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
             const int head = S.head;
             float cs_l = 0.f, cy_l = 0.f;
             if (hl > 0) {
-                const double r = lb_dots<KH, HL>(np, hl, head, m, H, [&](int k) { return g[k]; });
+                const float r = lb_dots<KH, HL>(np, hl, head, m, H, [&](int k) { return g[k]; });
                 Lb<KH>::solve(r, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
             }
             // iteration 1: d = -g; later: d = -H_diag g + history terms
@@ -578,12 +558,21 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
             const int niter = S.n_iter;
             const int hl = S.hist_len, head = S.head;
             // history pairs are stored from iteration 2 on (core/lbfgs_torch_parallel.py:636-690)
-            const int nv = niter > 1 ? hl : 0;
-            double gm, pc, ra, rb;
-            lb_update_dots<KH, HL>(np, nv, head, m, tf, H, g, gp, d, ra, rb, gm, pc);
-            const double ys = lane_d(ra, many_src<Lb<KH>::KA>(0));
-            const double yy = lane_d(ra, many_src<Lb<KH>::KA>(1));
+            float pa[2] = {0.f, 0.f};
+            double mu[2] = {0.0, 0.0};
+            for (int k = lane; k < np; k += WAVE) {
+                const float gk = g[k];
+                const float sk = d[k] * tf;
+                Lb<KH>::pair(pa, gk - gp[k], sk);
+                mu[0] = fmax(mu[0], fabs((double)gk));
+                mu[1] = fmax(mu[1], fabs((double)sk));
+            }
+            const double rp = wave_sum_many<2>(pa);
+            const double rmu = wave_max_many<2>(mu);
+            const double ys = lane_d(rp, many_src<2>(0)), yy = lane_d(rp, many_src<2>(1));
+            const double gm = lane_d(rmu, many_src<2>(0)), pc = lane_d(rmu, many_src<2>(1));
             if (niter > 1 && ys > 1e-10) {
+                const float r = lb_dots<KH, HL>(np, hl, head, m, H, [&](int k) { return g[k] - gp[k]; });
                 const int q = hl < m ? ring(head, hl, m) : head;
                 const int oq = q * H.stride;
                 for (int k = lane; k < np; k += WAVE) {
@@ -593,7 +582,7 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
                         H.s[oq + k] = d[k] * tf;
                     }
                 }
-                Lb<KH>::store(ra, rb, ys, yy, hl, head, m, q, SY, YY, S);
+                Lb<KH>::store(r, ys, yy, hl, head, m, q, SY, YY, S);
             }
             if (lane == 0) {
                 S.n_eval += 1;
@@ -668,8 +657,13 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
 // workgroup barrier ever waits on idle waves.  The arithmetic is that of k_relax (the
 // same helpers, the same per-lane order), so both kernels give the same bits and may
 // even continue each other's runs.
+// Waves per SIMD the one-wave kernel is compiled for: two for up to 5 rows (the register
+// budget of 256 then holds the state, the history and the evaluator), one above.
+template <int R>
+constexpr int wave_occupancy() { return R <= 5 ? 2 : 1; }
+
 template <int MODE, int R, int KH>
-__global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
+__global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs A) {
     extern __shared__ __align__(16) char smem[];
     __shared__ GeomT<float> G;
     __shared__ LbSlot S;
@@ -728,9 +722,10 @@ __global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
 #pragma unroll
     for (int j = 0; j < KH; ++j) {
         if (j < hl0) {
-            double acc[Lb<KH>::KD];
+            float acc[Lb<KH>::KD];
 #pragma unroll
-            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.0;
+            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
+            float pa[2] = {0.f, 0.f};
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 float sv[KH], yv[KH];
@@ -739,10 +734,12 @@ __global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
                     sv[i] = hs[i][r];
                     yv[i] = hy[i][r];
                 }
-                Lb<KH>::dots(acc, sv, yv, hy[j][r], j + 1);
+                Lb<KH>::dots(acc, sv, yv, hy[j][r], j);
+                Lb<KH>::pair(pa, hy[j][r], hs[j][r]);
             }
-            const double rr = wave_sum_many<Lb<KH>::KD>(acc);
-            Lb<KH>::rebuild_col(rr, j, head0, m, SY, YY);
+            const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
+            const double rp = wave_sum_many<2>(pa);
+            Lb<KH>::rebuild_col(rr, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), j, head0, m, SY, YY);
         }
     }
 
@@ -782,9 +779,9 @@ __global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
         const int head = S.head;
         float cs_l = 0.f, cy_l = 0.f;
         if (hl > 0) {
-            double acc[Lb<KH>::KD];
+            float acc[Lb<KH>::KD];
 #pragma unroll
-            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.0;
+            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 float sv[KH], yv[KH];
@@ -795,7 +792,7 @@ __global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
                 }
                 Lb<KH>::dots(acc, sv, yv, g[r], hl);
             }
-            const double rr = wave_sum_many<Lb<KH>::KD>(acc);
+            const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
             Lb<KH>::solve(rr, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
         }
         const float gamf = niter > 1 ? (float)S.H : 1.f;
@@ -857,39 +854,40 @@ __global__ __launch_bounds__(WAVE) void k_relax_w(RelaxArgs A) {
             loss = wave_evaluate<MODE, R>(G, L, A, x, g, xs, gs, N, n, bacc, st_tp, st_pp, lm, cut_list,
                                           nb_valid);
         // ---- history update and convergence tests ----
-        const int nv = niter > 1 ? hl : 0;
-        double a[Lb<KH>::KA], bb[Lb<KH>::KB];
-#pragma unroll
-        for (int i = 0; i < Lb<KH>::KA; ++i) a[i] = 0.0;
-#pragma unroll
-        for (int i = 0; i < Lb<KH>::KB; ++i) bb[i] = 0.0;
+        float pa[2] = {0.f, 0.f};
         double mu[2] = {0.0, 0.0};
         float yn[R], sn[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            float sv[KH], yv[KH];
-#pragma unroll
-            for (int i = 0; i < KH; ++i) {
-                sv[i] = hs[i][r];
-                yv[i] = hy[i][r];
-            }
             const float gk = g[r];
             yn[r] = gk - gp[r];
             sn[r] = d[r] * tf;
-            Lb<KH>::upd(a, bb, yn[r], sn[r], sv, yv, nv);
+            Lb<KH>::pair(pa, yn[r], sn[r]);
             mu[0] = fmax(mu[0], fabs((double)gk));
             mu[1] = fmax(mu[1], fabs((double)sn[r]));
         }
+        const double rp = wave_sum_many<2>(pa);
         const double rmu = wave_max_many<2>(mu);
+        const double ys = lane_d(rp, many_src<2>(0)), yy = lane_d(rp, many_src<2>(1));
         const double gm = lane_d(rmu, many_src<2>(0)), pc = lane_d(rmu, many_src<2>(1));
-        const double ra = wave_sum_many<Lb<KH>::KA>(a);
-        const double rb = wave_sum_many<Lb<KH>::KB>(bb);
-        const double ys = lane_d(ra, many_src<Lb<KH>::KA>(0));
-        const double yy = lane_d(ra, many_src<Lb<KH>::KA>(1));
         if (niter > 1 && ys > 1e-10) {
             const int hl_now = S.hist_len;
+            float acc[Lb<KH>::KD];
+#pragma unroll
+            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float sv[KH], yv[KH];
+#pragma unroll
+                for (int i = 0; i < KH; ++i) {
+                    sv[i] = hs[i][r];
+                    yv[i] = hy[i][r];
+                }
+                Lb<KH>::dots(acc, sv, yv, yn[r], hl_now);
+            }
+            const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
             const int q = hl_now < m ? ring(head, hl_now, m) : head;
-            Lb<KH>::store(ra, rb, ys, yy, hl_now, head, m, q, SY, YY, S);
+            Lb<KH>::store(rr, ys, yy, hl_now, head, m, q, SY, YY, S);
             if (hl_now < m) {
 #pragma unroll
                 for (int i = 0; i < KH; ++i) {
