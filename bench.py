@@ -123,6 +123,85 @@ def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0):
     return out
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` recorded from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (calibrated by a plain copy) in the committed profile summaries, or None."""
+    import glob
+    key = f"traffic_bytes_per_launch[{kernel}]"
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                line = line.strip().strip("`")
+                if line.startswith(key):
+                    try:
+                        return float(line.split("=", 1)[1]), os.path.relpath(f, ROOT)
+                    except ValueError:
+                        pass
+    return None, None
+
+
+def bucket_roofline(b, sizes, opts, chunk, n_chunks, stream):
+    """Each size bucket's kernel launched on its own (same chunks as the batched run, one
+    bucket at a time, HIP events on the launch stream).  Algorithmic bytes per launch:
+    table gathers (8 corners x 4 B per pair looked up, counted by the kernel) plus, for
+    every instance still running at the start of the launch, its state x and gradient
+    read and written once (16 B per state element) and its L-BFGS history rows read at
+    the start and written at the end (2 x 4 B per element and row); within a launch the
+    state and history stay on chip.  Returns the dominant (longest) bucket's figures and
+    a per-bucket list."""
+    import torch
+    # load every bucket's kernel before timing (the first launch of a kernel pays for
+    # loading its code object on the host), then start over from the initial poses
+    for k in range(len(b.buckets())):
+        b.launch_bucket(k, 1, stream=stream)
+    stream.synchronize()
+    b.reset()
+    torch.cuda.synchronize()
+    n_state = 3 * sizes.astype(np.int64) + 3
+    order = b.order.cpu().numpy()
+    bk = b.buckets()
+    per = [{"ms": [], "lookups": 0, "bytes": 0.0, "n_max": int(nmax), "instances": int(cnt)}
+           for (_, cnt, nmax) in bk]
+    rec0 = b.slot_records()
+    with torch.cuda.stream(stream):
+        for c in range(n_chunks):
+            for k, (off, cnt, nmax) in enumerate(bk):
+                idx = order[off:off + cnt]
+                b.stats.zero_()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                b.launch_bucket(k, chunk, stream=stream)
+                e1.record(stream)
+                stream.synchronize()
+                rec1 = b.slot_records()
+                run = rec0[idx, 3] != 2
+                rows = rec0[idx, 2].astype(np.int64) + rec1[idx, 2].astype(np.int64)
+                per[k]["ms"].append(e0.elapsed_time(e1))
+                per[k]["lookups"] += int(b.stats[1].item())
+                per[k]["bytes"] += float(np.sum((n_state[idx] * 4 * (4 + 2 * rows))[run]))
+                rec0[idx] = rec1[idx]
+    for k, q in enumerate(per):
+        q["bytes"] += q["lookups"] * 32.0
+        q["kernel"] = f"pk::k_relax_w<2, {-(-(3 * q['n_max'] + 3) // 64)}, 8>"
+    dom = max(per, key=lambda q: sum(q["ms"]))
+    avg_s = float(np.mean(dom["ms"])) / 1e3
+    alg = dom["bytes"] / n_chunks
+    achieved = alg / avg_s / 1e9
+    traffic, src = profiled_traffic(dom["kernel"])
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 5), "traffic": traffic,
+            "traffic_source": src,
+            "kernel": dom["kernel"] + f" (table mode, size bucket of {dom['instances']} instances, "
+                                      f"N <= {dom['n_max']})",
+            "avg_launch_ms": round(avg_s * 1e3, 3), "alg_bytes_per_launch": alg,
+            "lookups_per_launch": dom["lookups"] / n_chunks,
+            "buckets": [{"kernel": q["kernel"], "instances": q["instances"], "n_max": q["n_max"],
+                         "avg_launch_ms": round(float(np.mean(q["ms"])), 3),
+                         "alg_bytes_per_launch": q["bytes"] / n_chunks,
+                         "achieved_GBps": round(q["bytes"] / n_chunks / (float(np.mean(q["ms"])) / 1e3) / 1e9, 2)}
+                        for q in per]}
+
+
 def cpu_baseline(geom, lut, sizes, xs, hs, opts, budget_s):
     """Same workload on the host: the FP64 oracle (per-instance L-BFGS, table overlap +
     boundary), OpenMP over instances on every host core, on a bounded sample."""
@@ -167,6 +246,10 @@ def main():
     ap.add_argument("--serve-plain", type=int, default=10, help="requests timed on the plain path")
     ap.add_argument("--serve-iters", type=int, default=2000)
     ap.add_argument("--no-overlap", action="store_true", help="skip the exact-kernel roofline")
+    ap.add_argument("--overlap-only", action="store_true",
+                    help="only the exact-kernel roofline (the launches profiled with rocprofv3)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the per-bucket kernel measurement (the launches profiled with rocprofv3)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
@@ -199,6 +282,9 @@ def main():
     info = _native.device_info()
     geom = random_polygon(0)
     set_geometry(geom)
+    if args.overlap_only:
+        print(json.dumps({"overlap_kernel": overlap_kernel_roofline(geom, dev)}))
+        return
     t0 = time.perf_counter()
     lut = LookupTable.build_exact(256, 256, 128, device=dev)
     torch.cuda.synchronize()
@@ -211,6 +297,9 @@ def main():
     stream = torch.cuda.Stream(device=dev)
     chunk = min(args.chunk, args.iters)
     n_chunks = -(-args.iters // chunk)
+    if args.roofline_only:
+        print(json.dumps({"roofline": bucket_roofline(b, sizes, opts, chunk, n_chunks, stream)}))
+        return
 
     # one graph = reset + all chunked launches of one relaxation of the batch
     with torch.cuda.stream(stream):
@@ -247,41 +336,8 @@ def main():
     rec = b.slot_records()
     loss = b.loss.cpu().numpy()
 
-    # per-launch kernel time (eager chunks, events on the launch stream) + counters
-    b.reset()
-    torch.cuda.synchronize()
-    b.stats.zero_()
-    evs = []
-    with torch.cuda.stream(stream):
-        for _ in range(n_chunks):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            b.launch(chunk, stream=stream)
-            e1.record(stream)
-            evs.append((e0, e1))
-    torch.cuda.synchronize()
-    launch_ms = [a.elapsed_time(c) for a, c in evs]
-    lookups = int(b.stats[1].item())
-    n_state = 3 * sizes.astype(np.int64) + 3
-    # Algorithmic memory bytes per chunk launch: table gathers (8 corners x 4 B per pair
-    # looked up) plus, for every instance still running at the chunk's start, its state x
-    # and gradient read and written once (16 B per state element) and its L-BFGS history
-    # (history x (s, y) rows) copied into LDS and back once (2 x 2 x history x 4 B per
-    # element); within a launch the history, gradient and poses stay in LDS.
-    it_i = rec[:, 0].astype(np.int64)
-    run_bytes = 0.0
-    for c in range(n_chunks):
-        act = (it_i > c * chunk) | (c == 0)
-        hist_rows = 0 if c == 0 else 2 * 2 * opts.history
-        run_bytes += float(np.sum(n_state[act]) * 4 * (4 + hist_rows))
-    alg_bytes = lookups * 32 + run_bytes
-    avg_launch_s = float(np.mean(launch_ms)) / 1e3
-    achieved = alg_bytes / n_chunks / avg_launch_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(achieved / 8000.0, 5), "traffic": None,
-                "kernel": "k_relax<MODE_LUT> (one chunk: the size-bucket launches of 100 iterations)",
-                "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-                "alg_bytes_per_launch": alg_bytes / n_chunks, "lookups_per_launch": lookups / n_chunks}
+    roofline = bucket_roofline(b, sizes, opts, chunk, n_chunks, stream)
+    log(f"roofline: {roofline}")
 
     # plain path: one launch per instance per iteration, same kernel and numerics
     plain = None

@@ -625,6 +625,23 @@ struct Evaluator {
         piece_pair(i, j, pi, pj, L.x[j] - L.x[i], L.y[j] - L.y[i], false);
     }
 
+    // True when piece pi of tree i and piece pj of tree j (shifted by dx, dy) are separated
+    // along the line through their bounding-circle centres (direction ddx, ddy) by more than
+    // 1e-4 radius: such a pair is far from contact, clips to nothing and has no SAT depth,
+    // so it contributes exactly zero and needs no clipping.
+    __device__ inline bool centre_axis_separated(int i, int j, int pi, int pj, T dx, T dy, T ddx,
+                                                 T ddy) const {
+        const int oa = (i * L.P + pi) * MAXV, ob = (j * L.P + pj) * MAXV;
+        T amax = -T(1e30), bmin = T(1e30);
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            amax = fmax(amax, L.vx[oa + v] * ddx + L.vy[oa + v] * ddy);
+            bmin = fmin(bmin, (L.vx[ob + v] + dx) * ddx + (L.vy[ob + v] + dy) * ddy);
+        }
+        const T gap = bmin - amax, mu = T(1e-4) * G.radius;
+        return gap > T(0) && gap * gap > mu * mu * (ddx * ddx + ddy * ddy);
+    }
+
     // Expand the tn tree pairs of tq into the piece pairs whose bounding circles meet,
     // clipping 64 at a time from the wave's piece-pair queue.
     __device__ inline void expand(const uint32_t* tq, int tn, uint32_t* pq, int& pn) {
@@ -652,6 +669,7 @@ struct Evaluator {
                 const T rr = G.cr[pi] + G.cr[pj];
                 const T ddx = cxb - cxa, ddy = cyb - cya;
                 pred = ddx * ddx + ddy * ddy <= rr * rr;
+                if (pred) pred = !centre_axis_separated(i, j, pi, pj, dx, dy, ddx, ddy);
                 entry = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
             }
             const uint64_t m = __ballot(pred);
@@ -677,7 +695,6 @@ struct Evaluator {
         if (m) {
             if (pred) tq[tn + __popcll(m & lanemask_lt())] = entry;
             tn += __popcll(m);
-            n_tp += __popcll(m);
             wave_sync();
             if (MODE == MODE_LUT) {
                 // the table queue uses both halves of the wave's queue space (2 QCAP entries)
@@ -717,6 +734,7 @@ struct Evaluator {
                     const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
                     pred = dx * dx + dy * dy <= G.cutoff_sq;
                 }
+                n_tp += __popcll(__ballot(pred));
                 push(pred, e, tq, tn, pq, pn);
             }
         } else {
@@ -767,6 +785,7 @@ struct Evaluator {
                             if (pl && q < L.nb_cap) L.nb[q] = entry;
                         }
                     }
+                    n_tp += __popcll(__ballot(pred));
                     push(pred, entry, tq, tn, pq, pn);
                     if (__ballot(alive) == 0) break;
                 }

@@ -198,6 +198,17 @@ class RelaxBatch:
         for e in joins:
             main.wait_event(e)
 
+    def buckets(self):
+        """Size buckets as (first position in the size order, count, largest tree count)."""
+        return list(self._buckets)
+
+    def launch_bucket(self, k: int, iters: int, stream=None) -> None:
+        """Advance only the instances of size bucket k (one kernel launch)."""
+        off, cnt, nmax = self._buckets[k]
+        blk = int(self.opts.block) or self._block_for(nmax)
+        self._launch_one(iters, stream, order=self.order[off:off + cnt], n_launch=cnt, n_tree_max=nmax,
+                         block=blk)
+
     def _block_for(self, n_max: int) -> int:
         from .autotune import choose_block
         return choose_block(self.opts.mode, n_max, self.geometry.n_pieces)
