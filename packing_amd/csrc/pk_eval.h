@@ -963,18 +963,23 @@ __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
     if (VERTS || GRID) __syncthreads();
 }
 
+// A tree whose bounding circle (radius plus a margin far above the rounding of the vertex
+// positions in boundary_tree) lies inside the square has no vertex outside: its boundary
+// cost and gradients are zero, as the vertex loop would find.
+template <typename T>
+__device__ inline bool boundary_inside(const GeomT<T>& G, T x, T y, T half, T hx, T hy) {
+    const T rx = fabs(x - hx), ry = fabs(y - hy);
+    const T margin = G.radius * T(1e-4) + T(1e-5) * (T(1) + fabs(x) + fabs(y) + fabs(hx) + fabs(hy) + half);
+    return rx + G.radius + margin <= half && ry + G.radius + margin <= half;
+}
+
 // Boundary term of one tree (core/pack_cuda.py:550-642): max over outline vertices of
 // the squared distance outside the square h = [size, ox, oy]; gradients w.r.t. the pose
 // and h at the arg-max vertex (strict >, first vertex on ties, 0 if inside).
 template <typename T>
 __device__ inline T boundary_tree(const GeomT<T>& G, T x, T y, T c, T s, T half, T hx, T hy,
                                   T (&g)[3], T (&gh)[3]) {
-    // A tree whose bounding circle (radius plus a margin far above the rounding of the
-    // vertex positions below) lies inside the square has no vertex outside: cost and
-    // gradients are zero, as the vertex loop would find.
-    const T rx = fabs(x - hx), ry = fabs(y - hy);
-    const T margin = G.radius * T(1e-4) + T(1e-5) * (T(1) + fabs(x) + fabs(y) + fabs(hx) + fabs(hy) + half);
-    if (rx + G.radius + margin <= half && ry + G.radius + margin <= half) {
+    if (boundary_inside(G, x, y, half, hx, hy)) {
         g[0] = g[1] = g[2] = T(0);
         gh[0] = gh[1] = gh[2] = T(0);
         return T(0);

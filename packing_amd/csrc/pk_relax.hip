@@ -102,16 +102,47 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     st_pp += ev.n_pp;
     const float half = x[3 * N] * 0.5f, hx = x[3 * N + 1], hy = x[3 * N + 2];
     unsigned long long lb[4] = {0ull, 0ull, 0ull, 0ull};
-    for (int i = threadIdx.x; i < N; i += BLOCK) {
-        float gb[3], gh[3];
-        const float cb = boundary_tree<float>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
-        lb[0] += Acc<float>::local(cb);
-        lb[1] += Acc<float>::local(gh[0]);
-        lb[2] += Acc<float>::local(gh[1]);
-        lb[3] += Acc<float>::local(gh[2]);
+    // Boundary term: trees well inside the square (most of them) have zero boundary cost and
+    // gradient and are finished in the first pass; the others are compacted into the wave's
+    // queue space (free after the pair evaluation) and evaluated 64 at a time, so the
+    // vertex loop runs on full rows.
+    {
+        const int lane = threadIdx.x % WAVE;
+        uint32_t* bq = L.wq + (threadIdx.x / WAVE) * 2 * QCAP;
+        int bn = 0;
+        auto finish = [&](int i) {
+            float gb[3], gh[3];
+            const float cb = boundary_tree<float>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
+            lb[0] += Acc<float>::local(cb);
+            lb[1] += Acc<float>::local(gh[0]);
+            lb[2] += Acc<float>::local(gh[1]);
+            lb[3] += Acc<float>::local(gh[2]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            g[3 * i + k] = p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * gb[k];
+            for (int k = 0; k < 3; ++k)
+                g[3 * i + k] = p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * gb[k];
+        };
+        for (int base = threadIdx.x - lane; base < N; base += BLOCK) {
+            const int i = base + lane;
+            bool near = false;
+            if (i < N) {
+                near = !boundary_inside(G, L.x[i], L.y[i], half, hx, hy);
+                if (!near) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        g[3 * i + k] = p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * 0.f;
+                }
+            }
+            const uint64_t m = __ballot(near);
+            if (near) bq[bn + __popcll(m & lanemask_lt())] = (uint32_t)i;
+            bn += __popcll(m);
+            wave_sync();
+            if (bn >= WAVE) {
+                finish((int)bq[bn - WAVE + lane]);
+                bn -= WAVE;
+                wave_sync();
+            }
+        }
+        if (lane < bn) finish((int)bq[lane]);
     }
     // integer sums: exact in any order
     if (__ballot((lb[0] | lb[1] | lb[2] | lb[3]) != 0ull)) {
