@@ -259,6 +259,29 @@ __device__ inline double wave_max_many(double (&v)[K]) {
     for (int s = LK; s < 6; ++s) r = fmax(r, shfl_xor_d(r, 32 >> s));
     return r;
 }
+// The same in float (max is exact, so the result equals the double version's).
+template <int K>
+__device__ inline float wave_max_many_f(float (&v)[K]) {
+    constexpr int LK = ilog2c(K);
+    static_assert((1 << LK) == K && K >= 2 && K <= 64, "K must be a power of two in [2, 64]");
+    const int lane = __lane_id();
+#pragma unroll
+    for (int s = 0; s < LK; ++s) {
+        const int bit = 32 >> s;
+        const int h = K >> (s + 1);
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int j = 0; j < h; ++j) {
+            float send, keep;
+            blend_pair_f(v[j], v[h + j], up, send, keep);
+            v[j] = fmaxf(keep, __shfl_xor(send, bit));
+        }
+    }
+    float r = v[0];
+#pragma unroll
+    for (int s = LK; s < 6; ++s) r = fmaxf(r, __shfl_xor(r, 32 >> s));
+    return r;
+}
 // Lane holding value j of a K-value butterfly result.
 template <int K>
 __device__ inline int many_src(int j) { return j << (6 - ilog2c(K)); }

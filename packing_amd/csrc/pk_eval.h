@@ -40,10 +40,13 @@ struct LutDev {
 // One table lookup split in two so that the loads of several lookups can be in flight
 // together: lut_fetch computes the cell and issues the loads (the two theta-neighbour
 // corners of each (x, y) node come in one 8-byte load), lut_combine interpolates.
+// Gathered corners of one lookup as loaded: theta pairs (bt, bt + 1) at the four (x, y)
+// corners; top selects node bt + 1 for both theta corners.  The selection is left to
+// lut_combine so that nothing waits on the loads before the interpolation.
 struct LutTap {
-    float v000, v001, v010, v011, v100, v101, v110, v111;
+    float2 p00, p01, p10, p11;
     float fx, fy, ft;
-    bool flip, in;
+    bool flip, in, top;
 };
 
 __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, LutTap& t) {
@@ -59,23 +62,14 @@ __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, L
     t.ft = gt - it0;
     // theta pair (it0, min(it0 + 1, nt - 1)) read as nodes (bt, bt + 1), bt <= nt - 2
     const int bt = min(it0, T_.nt - 2);
-    const bool top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
+    t.top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
     const float* __restrict__ v = T_.vals;
     const size_t r00 = ((size_t)ix0 * T_.ny + iy0) * T_.nt + bt, r01 = ((size_t)ix0 * T_.ny + iy1) * T_.nt + bt;
     const size_t r10 = ((size_t)ix1 * T_.ny + iy0) * T_.nt + bt, r11 = ((size_t)ix1 * T_.ny + iy1) * T_.nt + bt;
-    float2 p00, p01, p10, p11;
-    __builtin_memcpy(&p00, v + r00, 8);
-    __builtin_memcpy(&p01, v + r01, 8);
-    __builtin_memcpy(&p10, v + r10, 8);
-    __builtin_memcpy(&p11, v + r11, 8);
-    t.v000 = top ? p00.y : p00.x;
-    t.v001 = p00.y;
-    t.v010 = top ? p01.y : p01.x;
-    t.v011 = p01.y;
-    t.v100 = top ? p10.y : p10.x;
-    t.v101 = p10.y;
-    t.v110 = top ? p11.y : p11.x;
-    t.v111 = p11.y;
+    __builtin_memcpy(&t.p00, v + r00, 8);
+    __builtin_memcpy(&t.p01, v + r01, 8);
+    __builtin_memcpy(&t.p10, v + r10, 8);
+    __builtin_memcpy(&t.p11, v + r11, 8);
 }
 
 // Trilinear value and analytic gradient w.r.t. (x, y, theta) of the table
@@ -84,12 +78,16 @@ __device__ inline float lut_combine(const LutDev& T_, const LutTap& t, float (&g
     g[0] = g[1] = g[2] = 0.f;
     if (!t.in) return 0.f;
     const float fx = t.fx, fy = t.fy, ft = t.ft;
-    const float v00 = t.v000 * (1.f - ft) + t.v001 * ft, v01 = t.v010 * (1.f - ft) + t.v011 * ft;
-    const float v10 = t.v100 * (1.f - ft) + t.v101 * ft, v11 = t.v110 * (1.f - ft) + t.v111 * ft;
+    const float v000 = t.top ? t.p00.y : t.p00.x, v001 = t.p00.y;
+    const float v010 = t.top ? t.p01.y : t.p01.x, v011 = t.p01.y;
+    const float v100 = t.top ? t.p10.y : t.p10.x, v101 = t.p10.y;
+    const float v110 = t.top ? t.p11.y : t.p11.x, v111 = t.p11.y;
+    const float v00 = v000 * (1.f - ft) + v001 * ft, v01 = v010 * (1.f - ft) + v011 * ft;
+    const float v10 = v100 * (1.f - ft) + v101 * ft, v11 = v110 * (1.f - ft) + v111 * ft;
     const float v0 = v00 * (1.f - fy) + v01 * fy, v1 = v10 * (1.f - fy) + v11 * fy;
     const float val = v0 * (1.f - fx) + v1 * fx;
-    const float d0 = (t.v001 - t.v000) * (1.f - fy) + (t.v011 - t.v010) * fy;
-    const float d1 = (t.v101 - t.v100) * (1.f - fy) + (t.v111 - t.v110) * fy;
+    const float d0 = (v001 - v000) * (1.f - fy) + (v011 - v010) * fy;
+    const float d1 = (v101 - v100) * (1.f - fy) + (v111 - v110) * fy;
     g[0] = (v1 - v0) * T_.sx;
     g[1] = ((1.f - fx) * (v01 - v00) + fx * (v11 - v10)) * T_.sy;
     g[2] = (d0 * (1.f - fx) + d1 * fx) * T_.st;
@@ -489,7 +487,10 @@ __device__ inline T grid_inv_side(const GeomT<T>& G) {
     return T(1) / (T(1.01) * sqrt(G.cutoff_sq));
 }
 
-template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK>
+// PIPE (table mode): the gathers of one queue drain stay in flight until the next drain,
+// so the table latency overlaps the candidate walk in between (costs the registers of
+// two lookups per lane for the whole walk).
+template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK, bool PIPE = false>
 struct Evaluator {
     using AccT = Acc<T>;
     using A = typename AccT::type;
@@ -599,6 +600,31 @@ struct Evaluator {
             AccT::add(&L.g[3 * j + 2], gl[2]);
         }
     }
+    // Pipelined drains (PIPE): finish the lookups begun at the previous drain, then begin
+    // those of cnt <= 128 queue entries (two per lane); the last ones are finished by
+    // lut_flush.  Integer accumulation makes the result independent of the order.
+    LutPair pend_a, pend_b;
+    int pend_cnt = 0;
+    __device__ inline void lut_flush() {
+        const int lane = threadIdx.x % WAVE;
+        if (lane < pend_cnt) lut_end(pend_a);
+        if (lane + WAVE < pend_cnt) lut_end(pend_b);
+        pend_cnt = 0;
+    }
+    __device__ inline void lut_pairs_pipe(const uint32_t* q, int cnt) {
+        lut_flush();
+        const int lane = threadIdx.x % WAVE;
+        if (lane < cnt) {
+            const uint32_t e = q[lane];
+            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pend_a);
+        }
+        if (lane + WAVE < cnt) {
+            const uint32_t e = q[lane + WAVE];
+            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pend_b);
+        }
+        pend_cnt = cnt;
+        n_pp += cnt;
+    }
     // Lookups for cnt <= 128 queue entries, two per lane, with the table loads of both in
     // flight before either is interpolated.
     __device__ inline void lut_pairs(const uint32_t* q, int cnt) {
@@ -699,7 +725,10 @@ struct Evaluator {
             if (MODE == MODE_LUT) {
                 // the table queue uses both halves of the wave's queue space (2 QCAP entries)
                 if (tn >= 2 * WAVE) {
-                    lut_pairs(tq + tn - 2 * WAVE, 2 * WAVE);
+                    if (PIPE)
+                        lut_pairs_pipe(tq + tn - 2 * WAVE, 2 * WAVE);
+                    else
+                        lut_pairs(tq + tn - 2 * WAVE, 2 * WAVE);
                     tn -= 2 * WAVE;
                     wave_sync();
                 }
@@ -792,6 +821,7 @@ struct Evaluator {
             }
         }
         if (MODE == MODE_LUT) {
+            if (PIPE) lut_flush();
             lut_pairs(tq, tn);
         } else {
             expand(tq, tn, pq, pn);

@@ -48,7 +48,7 @@ __host__ __device__ inline int pad64(int n) { return (n + WAVE - 1) / WAVE * WAV
 // 1 rebuilds the neighbour list while evaluating, 2 evaluates from it (no cell grid).
 // Each tree's poses, sincos and boundary term are handled by the same thread, so the
 // staging needs no barrier between them.
-template <int MODE, int BLOCK>
+template <int MODE, int BLOCK, bool PIPE = false>
 __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
                                     const float* x, float* g, int N,
                                     unsigned long long* bacc /* [4] boundary accumulators */,
@@ -90,7 +90,7 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
         }
     }
     if (MODE != MODE_LUT || grid) __syncthreads();
-    Evaluator<float, MODE, true, false, BLOCK> ev(G, L);
+    Evaluator<float, MODE, true, false, BLOCK, PIPE> ev(G, L);
     ev.cax = ev.cay = ev.cbx = ev.cby = 0.f;
     ev.npd = 0;
     ev.only_self = 0;
@@ -229,16 +229,15 @@ struct Lb {
         a[0] = fmaf(yk, sk, a[0]);
         a[1] = fmaf(yk, yk, a[1]);
     }
-    // d_k = -gamma g_k + sum_i (cs_i s_i,k + cy_i y_i,k), in float, in this order
+    // d_k = -gamma g_k + sum_i (cs_i s_i,k + cy_i y_i,k), in float, in this order, over all
+    // KH rows: rows at or beyond the used count are zero, as are their coefficients.
     __device__ static inline float dir(float gamf, float gk, const float (&cs)[KH], const float (&cy)[KH],
-                                       const float (&sv)[KH], const float (&yv)[KH], int hl) {
+                                       const float (&sv)[KH], const float (&yv)[KH]) {
         float dk = -gamf * gk;
 #pragma unroll
         for (int i = 0; i < KH; ++i) {
-            if (i < hl) {
-                dk += cs[i] * sv[i];
-                dk += cy[i] * yv[i];
-            }
+            dk += cs[i] * sv[i];
+            dk += cy[i] * yv[i];
         }
         return dk;
     }
@@ -250,37 +249,55 @@ struct Lb {
     // the same direction as the two-loop recursion of core/lbfgs_torch_parallel.py:532-572.
     // The small matrices are indexed by ring slot, (head + i) mod m for chronological i.  The
     // hl x hl triangular solves run one row per lane.  Returns, in lane i < hl, the
-    // coefficients of d = -gamma g + sum_i cs_i s_i + cy_i y_i.
+    // coefficients of d = -gamma g + sum_i cs_i s_i + cy_i y_i.  a, b: s_i.g and y_i.g of
+    // this lane's row i = lane mod KH.
     __device__ static inline void solve(float r, int hl, int head, int m, double gam, const double* SY,
                                         const double* YY, const double* ro, float& cs_l, float& cy_l) {
+        const int li = __lane_id() & (KH - 1);
+        solve_ab(__shfl(r, many_src<KD>(li)), __shfl(r, many_src<KD>(KH + li)), hl, head, m, gam, SY, YY, ro,
+                 cs_l, cy_l);
+    }
+    __device__ static inline void solve_ab(float a, float b, int hl, int head, int m, double gam,
+                                           const double* SY, const double* YY, const double* ro, float& cs_l,
+                                           float& cy_l) {
         const int lane = __lane_id();
-        const int li = lane & (KH - 1);
-        const double a_l = (double)__shfl(r, many_src<KD>(li));
-        const double b_l = (double)__shfl(r, many_src<KD>(KH + li));
+        const double a_l = (double)a;
+        const double b_l = (double)b;
         const bool row = lane < hl;
         const int pl = ring(head, lane < m ? lane : 0, m);
-        double acc = a_l, u_l = 0.0, zacc = 0.0;
-#pragma unroll
-        for (int j = KH - 1; j >= 0; --j) {
-            const int pj = ring(head, j < m ? j : 0, m);
-            const double sy = SY[pl * m + pj], yy = YY[pl * m + pj], rj = ro[pj];
-            if (j < hl) {
-                const double uj = lane_d(acc, j) * rj;
-                if (lane == j) u_l = uj;
-                if (lane < j) acc -= sy * uj;
-                if (row) zacc += yy * uj;
-            }
-        }
-        const double z_l = row ? SY[pl * m + pl] * u_l + gam * zacc - gam * b_l : 0.0;
-        double accw = z_l, w_l = 0.0;
+        // all matrix entries this lane needs, loaded before the dependent chains: row pl of
+        // S^T Y and Y^T Y, column pl of S^T Y, and 1 / s_j.y_j
+        double syr[KH], yyr[KH], syc[KH], rj[KH];
 #pragma unroll
         for (int j = 0; j < KH; ++j) {
             const int pj = ring(head, j < m ? j : 0, m);
-            const double sy = SY[pj * m + pl], rj = ro[pj];
+            syr[j] = SY[pl * m + pj];
+            yyr[j] = YY[pl * m + pj];
+            syc[j] = SY[pj * m + pl];
+            rj[j] = ro[pj];
+        }
+        const double dl = SY[pl * m + pl];
+        // Lane i's running value is read once, at step j = i; the updates of lanes at or
+        // beyond j (outside the triangle) come after that read or belong to unused rows, so
+        // they need no mask.
+        double acc = a_l, u_l = 0.0, zacc = 0.0;
+#pragma unroll
+        for (int j = KH - 1; j >= 0; --j) {
             if (j < hl) {
-                const double wj = lane_d(accw, j) * rj;
+                const double uj = lane_d(acc, j) * rj[j];
+                if (lane == j) u_l = uj;
+                acc -= syr[j] * uj;
+                zacc += yyr[j] * uj;
+            }
+        }
+        const double z_l = row ? dl * u_l + gam * zacc - gam * b_l : 0.0;
+        double accw = z_l, w_l = 0.0;
+#pragma unroll
+        for (int j = 0; j < KH; ++j) {
+            if (j < hl) {
+                const double wj = lane_d(accw, j) * rj[j];
                 if (lane == j) w_l = wj;
-                if (row && lane > j) accw -= sy * wj;
+                accw -= syc[j] * wj;
             }
         }
         cs_l = row ? (float)(-w_l) : 0.f;
@@ -292,10 +309,16 @@ struct Lb {
     // the diagonal from the pair's own sums, H_diag, ro and the ring counters.
     __device__ static inline void store(float r, double ys, double yy, int hl, int head, int m, int q,
                                         double* SY, double* YY, LbSlot& S) {
+        const int li = __lane_id() & (KH - 1);
+        store_v(__shfl(r, many_src<KD>(li)), __shfl(r, many_src<KD>(KH + li)), ys, yy, hl, head, m, q, SY, YY,
+                S);
+    }
+    // The same with this lane's s_i.y_new and y_i.y_new (row i = lane) given.
+    __device__ static inline void store_v(float sy_l, float yy_l, double ys, double yy, int hl, int head, int m,
+                                          int q, double* SY, double* YY, LbSlot& S) {
         const int lane = __lane_id();
-        const int li = lane & (KH - 1);
-        const double syc = (double)__shfl(r, many_src<KD>(li));
-        const double yyr = (double)__shfl(r, many_src<KD>(KH + li));
+        const double syc = (double)sy_l;
+        const double yyr = (double)yy_l;
         if (lane < hl) {
             const int pi = ring(head, lane, m);
             if (pi != q) {
@@ -402,7 +425,7 @@ __device__ inline void lb_direction(int np, int hl, int head, int m, float gamf,
         float sv[KH], yv[KH];
         hist_elem<KH, HL>(H, off, k, hl, sv, yv);
         const float gk = g[k];
-        const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv, hl);
+        const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv);
         d[k] = dk;
         sv2[0] += (double)gk * (double)dk;
         sv2[1] += fabs((double)gk);
@@ -659,6 +682,15 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     }
 }
 
+// Waves per SIMD the one-wave kernel is compiled for: two for up to 5 rows (the register
+// budget of 256 then holds the state, the history and the evaluator), one above.
+template <int R>
+constexpr int wave_occupancy() { return R <= 5 ? 2 : 1; }
+// Table gathers kept in flight across the candidate walk: only where one wave has the
+// SIMD to itself (a second wave hides the latency otherwise) and registers are left.
+template <int R>
+constexpr bool wave_pipe() { return wave_occupancy<R>() == 1; }
+
 // Evaluation of the register state x into g for k_relax_w (through the LDS vectors xs, gs).
 template <int MODE, int R>
 __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
@@ -669,7 +701,8 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
 #pragma unroll
     for (int r = 0; r < R; ++r) xs[lane + WAVE * r] = x[r];
     __syncthreads();
-    const double loss = relax_eval<MODE, WAVE>(G, L, A, xs, gs, N, bacc, st_tp, st_pp, lm, cut_list);
+    const double loss = relax_eval<MODE, WAVE, wave_pipe<R>()>(G, L, A, xs, gs, N, bacc, st_tp, st_pp, lm,
+                                                               cut_list);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -688,11 +721,6 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
 // workgroup barrier ever waits on idle waves.  The arithmetic is that of k_relax (the
 // same helpers, the same per-lane order), so both kernels give the same bits and may
 // even continue each other's runs.
-// Waves per SIMD the one-wave kernel is compiled for: two for up to 5 rows (the register
-// budget of 256 then holds the state, the history and the evaluator), one above.
-template <int R>
-constexpr int wave_occupancy() { return R <= 5 ? 2 : 1; }
-
 template <int MODE, int R, int KH>
 __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs A) {
     extern __shared__ __align__(16) char smem[];
@@ -802,6 +830,35 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         __syncthreads();
     }
 
+    // s_i.g and y_i.g of this lane's history row (lane mod KH) for the next direction: from
+    // the stored history here, then from the pass that follows each evaluation
+    const int li = lane & (KH - 1);
+    float a_l = 0.f, b_l = 0.f;
+    // rows of this lane that hold a tree centre coordinate (x or y, not theta or h):
+    // k = lane + 64 r with k < 3N and k mod 3 != 2 (64 = 1 mod 3)
+    unsigned cmask = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (lane + WAVE * r < 3 * N && (lane + r) % 3 != 2) cmask |= 1u << r;
+    if (S.status == ST_RUN && hl0 > 0) {
+        float acc[Lb<KH>::KD];
+#pragma unroll
+        for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float sv[KH], yv[KH];
+#pragma unroll
+            for (int i = 0; i < KH; ++i) {
+                sv[i] = hs[i][r];
+                yv[i] = hy[i][r];
+            }
+            Lb<KH>::dots(acc, sv, yv, g[r], hl0);
+        }
+        const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
+        a_l = __shfl(rr, many_src<Lb<KH>::KD>(li));
+        b_l = __shfl(rr, many_src<Lb<KH>::KD>(KH + li));
+    }
+
     for (int it = 0; it < p.iters; ++it) {
         if (S.status == ST_DONE) break;
         // ---- direction, step length, directional-derivative test, step ----
@@ -809,23 +866,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         const int hl = niter > 1 ? S.hist_len : 0;
         const int head = S.head;
         float cs_l = 0.f, cy_l = 0.f;
-        if (hl > 0) {
-            float acc[Lb<KH>::KD];
-#pragma unroll
-            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                float sv[KH], yv[KH];
-#pragma unroll
-                for (int i = 0; i < KH; ++i) {
-                    sv[i] = hs[i][r];
-                    yv[i] = hy[i][r];
-                }
-                Lb<KH>::dots(acc, sv, yv, g[r], hl);
-            }
-            const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
-            Lb<KH>::solve(rr, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
-        }
+        if (hl > 0) Lb<KH>::solve_ab(a_l, b_l, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
         const float gamf = niter > 1 ? (float)S.H : 1.f;
         float cs[KH], cy[KH];
 #pragma unroll
@@ -833,7 +874,8 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             cs[i] = lane_f(cs_l, i);
             cy[i] = lane_f(cy_l, i);
         }
-        double sv2[2] = {0.0, 0.0}, mv[2] = {0.0, 0.0};
+        double sv2[2] = {0.0, 0.0};
+        float mv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             float sv[KH], yv[KH];
@@ -843,17 +885,20 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
                 yv[i] = hy[i][r];
             }
             const float gk = g[r];
-            const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv, hl);
+            const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv);
             d[r] = dk;
             sv2[0] += (double)gk * (double)dk;
             sv2[1] += fabs((double)gk);
-            mv[0] = fmax(mv[0], fabs((double)dk));
-            mv[1] = fmax(mv[1], fabs((double)x[r]));
+            mv[0] = fmaxf(mv[0], fabsf(dk));
+            const bool c = (cmask >> r) & 1u;
+            mv[1] = fmaxf(mv[1], c ? fabsf(dk) : 0.f);
+            mv[2] = fmaxf(mv[2], c ? fabsf(x[r]) : 0.f);
         }
         const double rs = wave_sum_many<2>(sv2);
-        const double rm = wave_max_many<2>(mv);
+        const float rm = wave_max_many_f<4>(mv);
         const double gtd = lane_d(rs, many_src<2>(0)), gsum = lane_d(rs, many_src<2>(1));
-        const double dmax = lane_d(rm, many_src<2>(0)), xmax = lane_d(rm, many_src<2>(1));
+        const double dmax = (double)lane_f(rm, many_src<4>(0));
+        const double dcen = (double)lane_f(rm, many_src<4>(1)), xcen = (double)lane_f(rm, many_src<4>(2));
         double t = niter == 1 ? fmin(1.0, 1.0 / gsum) * (double)p.lr : (double)p.lr;
         const double pc0 = dmax * fabs(t);
         if (p.max_step > 0.f && pc0 > (double)p.max_step) t *= (double)p.max_step / pc0;
@@ -872,9 +917,10 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             gp[r] = g[r];
             x[r] = x[r] + tf * d[r];
         }
-        // neighbour-list validity: as k_relax
-        const double step = dmax * fabs(t);
-        vdisp += 1.4142136 * (step * (1.0 + 1e-6) + (xmax + step) * 1.2e-7);
+        // neighbour-list validity as in k_relax, from the centre coordinates only (theta and
+        // h steps move no centre)
+        const double step = dcen * fabs(t);
+        vdisp += 1.4142136 * (step * (1.0 + 1e-6) + (xcen + step) * 1.2e-7);
         int lm = (nb_valid && 2.0 * vdisp < 0.9 * (double)skin) ? 2 : 1;
         if (lm == 1) vdisp = 0.0;
         if (lane == 0) S.n_iter = niter;
@@ -885,8 +931,14 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             loss = wave_evaluate<MODE, R>(G, L, A, x, g, xs, gs, N, n, bacc, st_tp, st_pp, lm, cut_list,
                                           nb_valid);
         // ---- history update and convergence tests ----
-        float pa[2] = {0.f, 0.f};
-        double mu[2] = {0.0, 0.0};
+        // One pass over the history rows i in use gives s_i.g and y_i.g for the next
+        // direction and s_i.y_new and y_i.y_new for the new S^T Y column and Y^T Y row; the
+        // new pair's own products s.g, y.g, y.s, y.y; and max |g|, max |s|.  Each sum has
+        // the per-lane partials and the reduction tree of the separate passes of k_relax.
+        constexpr int KF = 4 * KH;
+        float acc[KF], an[4] = {0.f, 0.f, 0.f, 0.f}, pa[2] = {0.f, 0.f}, mu[2] = {0.f, 0.f};
+#pragma unroll
+        for (int i = 0; i < KF; ++i) acc[i] = 0.f;
         float yn[R], sn[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -894,31 +946,30 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             yn[r] = gk - gp[r];
             sn[r] = d[r] * tf;
             Lb<KH>::pair(pa, yn[r], sn[r]);
-            mu[0] = fmax(mu[0], fabs((double)gk));
-            mu[1] = fmax(mu[1], fabs((double)sn[r]));
-        }
-        const double rp = wave_sum_many<2>(pa);
-        const double rmu = wave_max_many<2>(mu);
-        const double ys = lane_d(rp, many_src<2>(0)), yy = lane_d(rp, many_src<2>(1));
-        const double gm = lane_d(rmu, many_src<2>(0)), pc = lane_d(rmu, many_src<2>(1));
-        if (niter > 1 && ys > 1e-10) {
-            const int hl_now = S.hist_len;
-            float acc[Lb<KH>::KD];
+            mu[0] = fmaxf(mu[0], fabsf(gk));
+            mu[1] = fmaxf(mu[1], fabsf(sn[r]));
 #pragma unroll
-            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                float sv[KH], yv[KH];
-#pragma unroll
-                for (int i = 0; i < KH; ++i) {
-                    sv[i] = hs[i][r];
-                    yv[i] = hy[i][r];
-                }
-                Lb<KH>::dots(acc, sv, yv, yn[r], hl_now);
+            for (int i = 0; i < KH; ++i) {
+                acc[i] = fmaf(hs[i][r], gk, acc[i]);
+                acc[KH + i] = fmaf(hy[i][r], gk, acc[KH + i]);
+                acc[2 * KH + i] = fmaf(hs[i][r], yn[r], acc[2 * KH + i]);
+                acc[3 * KH + i] = fmaf(hy[i][r], yn[r], acc[3 * KH + i]);
             }
-            const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
+            an[0] = fmaf(sn[r], gk, an[0]);
+            an[1] = fmaf(yn[r], gk, an[1]);
+        }
+        const float rf = wave_sum_many_f<KF>(acc);
+        const float rn = wave_sum_many_f<4>(an);
+        const double rp = wave_sum_many<2>(pa);
+        const float rmu = wave_max_many_f<2>(mu);
+        const double ys = lane_d(rp, many_src<2>(0)), yy = lane_d(rp, many_src<2>(1));
+        const double gm = (double)lane_f(rmu, many_src<2>(0)), pc = (double)lane_f(rmu, many_src<2>(1));
+        const int hl_now = S.hist_len;
+        const bool acc_ok = niter > 1 && ys > 1e-10;
+        if (acc_ok) {
             const int q = hl_now < m ? ring(head, hl_now, m) : head;
-            Lb<KH>::store(rr, ys, yy, hl_now, head, m, q, SY, YY, S);
+            Lb<KH>::store_v(__shfl(rf, many_src<KF>(2 * KH + li)), __shfl(rf, many_src<KF>(3 * KH + li)), ys, yy,
+                            hl_now, head, m, q, SY, YY, S);
             if (hl_now < m) {
 #pragma unroll
                 for (int i = 0; i < KH; ++i) {
@@ -952,6 +1003,17 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
                     }
                 }
             }
+        }
+        {
+            // row li of the history the next direction sees: the old row li, or li + 1 when
+            // the oldest was dropped, or the new pair
+            const bool full = hl_now >= m;
+            const bool is_new = acc_ok && li == (full ? m - 1 : hl_now);
+            const int src = min((acc_ok && full) ? li + 1 : li, KH - 1);
+            const float an_s = lane_f(rn, many_src<4>(0)), an_y = lane_f(rn, many_src<4>(1));
+            const float ao = __shfl(rf, many_src<KF>(src)), bo = __shfl(rf, many_src<KF>(KH + src));
+            a_l = is_new ? an_s : ao;
+            b_l = is_new ? an_y : bo;
         }
         if (lane == 0) {
             S.n_eval += 1;
