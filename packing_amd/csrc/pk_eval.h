@@ -119,7 +119,10 @@ __device__ inline void load_geom(GeomT<T>& G, const GeomT<T>* __restrict__ src) 
 
 constexpr int TP_CAP = 1024;   // block-wide tree-pair list capacity (periodic images)
 constexpr int PP_CAP = 2048;   // block-wide piece-pair list capacity (periodic images)
-constexpr int QCAP = 128;      // per-wave queue capacity (entries), tree pairs and piece pairs
+constexpr int QCAP = 160;      // per-wave queue capacity (entries), tree pairs and piece pairs
+constexpr int LUT_UMAX = 4;    // most table lookups per lane per drain (the queue holds
+                               // 2 QCAP >= 64 LUT_UMAX + 63 entries)
+static_assert(2 * QCAP >= WAVE * LUT_UMAX + WAVE - 1, "table queue capacity");
 constexpr int MAX_WAVES = 4;   // workgroups of at most 256 threads
 
 // Cell grid for candidate pairs: hashed buckets of square cells whose side is at least
@@ -487,11 +490,11 @@ __device__ inline T grid_inv_side(const GeomT<T>& G) {
     return T(1) / (T(1.01) * sqrt(G.cutoff_sq));
 }
 
-// PIPE (table mode): the gathers of one queue drain stay in flight until the next drain,
-// so the table latency overlaps the candidate walk in between (costs the registers of
-// two lookups per lane for the whole walk).
-template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK, bool PIPE = false>
+// LU: table lookups per lane per queue drain, all in flight together (more hide more of
+// the gather latency where a wave has its SIMD to itself, at the cost of registers).
+template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK, int LU = 2>
 struct Evaluator {
+    static_assert(LU >= 1 && LU <= LUT_UMAX, "lookups per lane");
     using AccT = Acc<T>;
     using A = typename AccT::type;
     static constexpr int NW = BLOCK / WAVE;
@@ -600,47 +603,21 @@ struct Evaluator {
             AccT::add(&L.g[3 * j + 2], gl[2]);
         }
     }
-    // Pipelined drains (PIPE): finish the lookups begun at the previous drain, then begin
-    // those of cnt <= 128 queue entries (two per lane); the last ones are finished by
-    // lut_flush.  Integer accumulation makes the result independent of the order.
-    LutPair pend_a, pend_b;
-    int pend_cnt = 0;
-    __device__ inline void lut_flush() {
-        const int lane = threadIdx.x % WAVE;
-        if (lane < pend_cnt) lut_end(pend_a);
-        if (lane + WAVE < pend_cnt) lut_end(pend_b);
-        pend_cnt = 0;
-    }
-    __device__ inline void lut_pairs_pipe(const uint32_t* q, int cnt) {
-        lut_flush();
-        const int lane = threadIdx.x % WAVE;
-        if (lane < cnt) {
-            const uint32_t e = q[lane];
-            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pend_a);
-        }
-        if (lane + WAVE < cnt) {
-            const uint32_t e = q[lane + WAVE];
-            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pend_b);
-        }
-        pend_cnt = cnt;
-        n_pp += cnt;
-    }
-    // Lookups for cnt <= 128 queue entries, two per lane, with the table loads of both in
-    // flight before either is interpolated.
+    // Lookups for cnt <= 64 LU queue entries, LU per lane, with the table loads of all of
+    // them in flight before any is interpolated.
     __device__ inline void lut_pairs(const uint32_t* q, int cnt) {
         const int lane = threadIdx.x % WAVE;
-        const bool a = lane < cnt, b = lane + WAVE < cnt;
-        LutPair pa, pb;
-        if (a) {
-            const uint32_t e = q[lane];
-            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pa);
+        LutPair pr[LU];
+#pragma unroll
+        for (int u = 0; u < LU; ++u) {
+            if (lane + WAVE * u < cnt) {
+                const uint32_t e = q[lane + WAVE * u];
+                lut_begin(e & 0xfff, (e >> 12) & 0xfff, pr[u]);
+            }
         }
-        if (b) {
-            const uint32_t e = q[lane + WAVE];
-            lut_begin(e & 0xfff, (e >> 12) & 0xfff, pb);
-        }
-        if (a) lut_end(pa);
-        if (b) lut_end(pb);
+#pragma unroll
+        for (int u = 0; u < LU; ++u)
+            if (lane + WAVE * u < cnt) lut_end(pr[u]);
         n_pp += cnt;
     }
 
@@ -724,12 +701,9 @@ struct Evaluator {
             wave_sync();
             if (MODE == MODE_LUT) {
                 // the table queue uses both halves of the wave's queue space (2 QCAP entries)
-                if (tn >= 2 * WAVE) {
-                    if (PIPE)
-                        lut_pairs_pipe(tq + tn - 2 * WAVE, 2 * WAVE);
-                    else
-                        lut_pairs(tq + tn - 2 * WAVE, 2 * WAVE);
-                    tn -= 2 * WAVE;
+                if (tn >= LU * WAVE) {
+                    lut_pairs(tq + tn - LU * WAVE, LU * WAVE);
+                    tn -= LU * WAVE;
                     wave_sync();
                 }
             } else if (tn >= WAVE) {
@@ -752,19 +726,29 @@ struct Evaluator {
         int tn = 0, pn = 0;
         const int N = L.N;
         if (list_mode == 2) {
+            // four rows of 64 list entries per step, all loads issued before the tests
+            constexpr int U = 4;
             const int nl = L.wcnt[8];
-            for (int base = wave * WAVE; base < nl; base += BLOCK) {
-                const int k = base + lane;
-                bool pred = false;
-                uint32_t e = 0;
-                if (k < nl) {
-                    e = L.nb[k];
-                    const int i = e & 0xfff, j = (e >> 12) & 0xfff;
-                    const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-                    pred = dx * dx + dy * dy <= G.cutoff_sq;
+            for (int base = wave * WAVE; base < nl; base += U * BLOCK) {
+                uint32_t e[U];
+                bool pred[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = base + u * BLOCK + lane;
+                    e[u] = k < nl ? L.nb[k] : 0u;
                 }
-                n_tp += __popcll(__ballot(pred));
-                push(pred, e, tq, tn, pq, pn);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int k = base + u * BLOCK + lane;
+                    const int i = e[u] & 0xfff, j = (e[u] >> 12) & 0xfff;
+                    const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                    pred[u] = k < nl && dx * dx + dy * dy <= G.cutoff_sq;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    n_tp += __popcll(__ballot(pred[u]));
+                    push(pred[u], e[u], tq, tn, pq, pn);
+                }
             }
         } else {
             for (int base = wave * WAVE; base < N; base += BLOCK) {
@@ -821,7 +805,6 @@ struct Evaluator {
             }
         }
         if (MODE == MODE_LUT) {
-            if (PIPE) lut_flush();
             lut_pairs(tq, tn);
         } else {
             expand(tq, tn, pq, pn);

@@ -48,7 +48,7 @@ __host__ __device__ inline int pad64(int n) { return (n + WAVE - 1) / WAVE * WAV
 // 1 rebuilds the neighbour list while evaluating, 2 evaluates from it (no cell grid).
 // Each tree's poses, sincos and boundary term are handled by the same thread, so the
 // staging needs no barrier between them.
-template <int MODE, int BLOCK, bool PIPE = false>
+template <int MODE, int BLOCK, int LU = 2>
 __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
                                     const float* x, float* g, int N,
                                     unsigned long long* bacc /* [4] boundary accumulators */,
@@ -90,7 +90,7 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
         }
     }
     if (MODE != MODE_LUT || grid) __syncthreads();
-    Evaluator<float, MODE, true, false, BLOCK, PIPE> ev(G, L);
+    Evaluator<float, MODE, true, false, BLOCK, LU> ev(G, L);
     ev.cax = ev.cay = ev.cbx = ev.cby = 0.f;
     ev.npd = 0;
     ev.only_self = 0;
@@ -337,6 +337,35 @@ struct Lb {
         }
     }
 
+    // Chronological index of ring slot p, (p - head) mod m, for p < m; p itself beyond.
+    __device__ static inline int chron(int p, int head, int m) {
+        return p < m ? (p >= head ? p - head : p - head + m) : p;
+    }
+    // Direction coefficients by ring slot: lane p receives the coefficient of chronological
+    // row chron(p) (zero for rows not in use, as solve leaves them).
+    __device__ static inline float to_slot(float c_l, int head, int m) {
+        return __shfl(c_l, chron(__lane_id(), head, m));
+    }
+    // Column of ring slot p from the dots butterfly r of y_p against every slot (value i:
+    // s_i.y_p, KH + i: y_i.y_p) and the pair sums of (s_p, y_p): the entries of the slots
+    // older than p, as store() wrote them.
+    __device__ static inline void rebuild_col_slot(float r, double dsy, double dyy, int p, int hl, int head,
+                                                   int m, double* SY, double* YY) {
+        const int lane = __lane_id();
+        const int li = lane & (KH - 1);
+        const double sy = (double)__shfl(r, many_src<KD>(li));
+        const double yy = (double)__shfl(r, many_src<KD>(KH + li));
+        if (lane < hl && chron(lane, head, m) < chron(p, head, m)) {
+            SY[lane * m + p] = sy;
+            YY[lane * m + p] = yy;
+            YY[p * m + lane] = yy;
+        }
+        if (lane == 0) {
+            SY[p * m + p] = dsy;
+            YY[p * m + p] = dyy;
+        }
+    }
+
     // Column j (chronological) of S^T Y (upper) and Y^T Y from the dots butterfly r of y_j
     // and the pair sums of (s_j, y_j), as store() wrote them.
     __device__ static inline void rebuild_col(float r, double dsy, double dyy, int j, int head, int m,
@@ -407,19 +436,22 @@ __device__ inline void lb_rebuild(int np, int hl, int head, int m, const Hist<HL
 }
 
 // Direction pass: d written to LDS, with g.d and sum |g| (one butterfly), max |d| and
-// max |x| (another).  Wave 0 only.
+// max |x| (another).  The history terms are added in ring-slot order.  Wave 0 only.
 template <int KH, bool HL>
 __device__ inline void lb_direction(int np, int hl, int head, int m, float gamf, const Hist<HL>& H,
                                     const float* g, const float* x, float* d, float cs_l, float cy_l,
                                     double& gtd, double& gsum, double& dmax, double& xmax) {
+    // rows by ring slot (slots at or beyond hl unused), coefficients permuted to match
+    const float cs_s = Lb<KH>::to_slot(cs_l, head, m), cy_s = Lb<KH>::to_slot(cy_l, head, m);
     float cs[KH], cy[KH];
 #pragma unroll
     for (int i = 0; i < KH; ++i) {
-        cs[i] = lane_f(cs_l, i);
-        cy[i] = lane_f(cy_l, i);
+        cs[i] = lane_f(cs_s, i);
+        cy[i] = lane_f(cy_s, i);
     }
     int off[KH];
-    hist_offsets<KH, HL>(H, head, m, off);
+#pragma unroll
+    for (int i = 0; i < KH; ++i) off[i] = (i < m ? i : 0) * H.stride;
     double sv2[2] = {0.0, 0.0}, mv[2] = {0.0, 0.0};
     for (int k = threadIdx.x; k < np; k += WAVE) {
         float sv[KH], yv[KH];
@@ -686,10 +718,10 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
 // budget of 256 then holds the state, the history and the evaluator), one above.
 template <int R>
 constexpr int wave_occupancy() { return R <= 5 ? 2 : 1; }
-// Table gathers kept in flight across the candidate walk: only where one wave has the
-// SIMD to itself (a second wave hides the latency otherwise) and registers are left.
+// Table lookups per lane in flight at once: four where one wave has the SIMD to itself,
+// two where a second wave hides the gather latency and registers are scarcer.
 template <int R>
-constexpr bool wave_pipe() { return wave_occupancy<R>() == 1; }
+constexpr int wave_lookups() { return wave_occupancy<R>() == 1 ? 4 : 2; }
 
 // Evaluation of the register state x into g for k_relax_w (through the LDS vectors xs, gs).
 template <int MODE, int R>
@@ -701,8 +733,8 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
 #pragma unroll
     for (int r = 0; r < R; ++r) xs[lane + WAVE * r] = x[r];
     __syncthreads();
-    const double loss = relax_eval<MODE, WAVE, wave_pipe<R>()>(G, L, A, xs, gs, N, bacc, st_tp, st_pp, lm,
-                                                               cut_list);
+    const double loss = relax_eval<MODE, WAVE, wave_lookups<R>()>(G, L, A, xs, gs, N, bacc, st_tp, st_pp, lm,
+                                                                  cut_list);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -772,15 +804,15 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
 #pragma unroll
         for (int i = 0; i < KH; ++i) {
             const bool use = in && i < hl0;
-            const long long o = use ? (long long)ring(head0, i, m) * n + k : 0;
+            const long long o = use ? (long long)i * n + k : 0;
             hs[i][r] = use ? Sh[o] : 0.f;
             hy[i][r] = use ? Yh[o] : 0.f;
         }
     }
-    // rebuild S^T Y and Y^T Y from the stored history
+    // rebuild S^T Y and Y^T Y from the stored history, one ring slot (column) at a time
 #pragma unroll
-    for (int j = 0; j < KH; ++j) {
-        if (j < hl0) {
+    for (int pc = 0; pc < KH; ++pc) {
+        if (pc < hl0) {
             float acc[Lb<KH>::KD];
 #pragma unroll
             for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
@@ -793,12 +825,13 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
                     sv[i] = hs[i][r];
                     yv[i] = hy[i][r];
                 }
-                Lb<KH>::dots(acc, sv, yv, hy[j][r], j);
-                Lb<KH>::pair(pa, hy[j][r], hs[j][r]);
+                Lb<KH>::dots(acc, sv, yv, hy[pc][r], KH);
+                Lb<KH>::pair(pa, hy[pc][r], hs[pc][r]);
             }
             const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
             const double rp = wave_sum_many<2>(pa);
-            Lb<KH>::rebuild_col(rr, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), j, head0, m, SY, YY);
+            Lb<KH>::rebuild_col_slot(rr, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), pc, hl0, head0, m,
+                                     SY, YY);
         }
     }
 
@@ -830,8 +863,10 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         __syncthreads();
     }
 
-    // s_i.g and y_i.g of this lane's history row (lane mod KH) for the next direction: from
-    // the stored history here, then from the pass that follows each evaluation
+    // History rows live in registers by ring slot (row i of hs, hy = slot i; slots at or
+    // beyond the used count are zero), so a new pair overwrites one slot and nothing moves.
+    // s_i.g and y_i.g of this lane's chronological history row (lane mod KH) for the next
+    // direction: from the stored history here, then from the pass after each evaluation.
     const int li = lane & (KH - 1);
     float a_l = 0.f, b_l = 0.f;
     // rows of this lane that hold a tree centre coordinate (x or y, not theta or h):
@@ -852,11 +887,12 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
                 sv[i] = hs[i][r];
                 yv[i] = hy[i][r];
             }
-            Lb<KH>::dots(acc, sv, yv, g[r], hl0);
+            Lb<KH>::dots(acc, sv, yv, g[r], KH);
         }
         const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
-        a_l = __shfl(rr, many_src<Lb<KH>::KD>(li));
-        b_l = __shfl(rr, many_src<Lb<KH>::KD>(KH + li));
+        const int ps = ring(head0, li < m ? li : 0, m);
+        a_l = __shfl(rr, many_src<Lb<KH>::KD>(ps));
+        b_l = __shfl(rr, many_src<Lb<KH>::KD>(KH + ps));
     }
 
     for (int it = 0; it < p.iters; ++it) {
@@ -868,11 +904,12 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         float cs_l = 0.f, cy_l = 0.f;
         if (hl > 0) Lb<KH>::solve_ab(a_l, b_l, hl, head, m, S.H, SY, YY, S.ro, cs_l, cy_l);
         const float gamf = niter > 1 ? (float)S.H : 1.f;
+        const float cs_s = Lb<KH>::to_slot(cs_l, head, m), cy_s = Lb<KH>::to_slot(cy_l, head, m);
         float cs[KH], cy[KH];
 #pragma unroll
         for (int i = 0; i < KH; ++i) {
-            cs[i] = lane_f(cs_l, i);
-            cy[i] = lane_f(cy_l, i);
+            cs[i] = lane_f(cs_s, i);
+            cy[i] = lane_f(cy_s, i);
         }
         double sv2[2] = {0.0, 0.0};
         float mv[4] = {0.f, 0.f, 0.f, 0.f};
@@ -966,52 +1003,30 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         const double gm = (double)lane_f(rmu, many_src<2>(0)), pc = (double)lane_f(rmu, many_src<2>(1));
         const int hl_now = S.hist_len;
         const bool acc_ok = niter > 1 && ys > 1e-10;
+        const bool full = hl_now >= m;
+        const int q = full ? head : hl_now;   // slot of the new pair (the oldest when full)
         if (acc_ok) {
-            const int q = hl_now < m ? ring(head, hl_now, m) : head;
-            Lb<KH>::store_v(__shfl(rf, many_src<KF>(2 * KH + li)), __shfl(rf, many_src<KF>(3 * KH + li)), ys, yy,
+            const int pl = ring(head, li < m ? li : 0, m);
+            Lb<KH>::store_v(__shfl(rf, many_src<KF>(2 * KH + pl)), __shfl(rf, many_src<KF>(3 * KH + pl)), ys, yy,
                             hl_now, head, m, q, SY, YY, S);
-            if (hl_now < m) {
 #pragma unroll
-                for (int i = 0; i < KH; ++i) {
-                    if (i == hl_now) {
+            for (int i = 0; i < KH; ++i) {
+                if (i == q) {
 #pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            hs[i][r] = sn[r];
-                            hy[i][r] = yn[r];
-                        }
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i + 1 < KH; ++i) {
-                    if (i + 1 < m) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            hs[i][r] = hs[i + 1][r];
-                            hy[i][r] = hy[i + 1][r];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < KH; ++i) {
-                    if (i == m - 1) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            hs[i][r] = sn[r];
-                            hy[i][r] = yn[r];
-                        }
+                    for (int r = 0; r < R; ++r) {
+                        hs[i][r] = sn[r];
+                        hy[i][r] = yn[r];
                     }
                 }
             }
         }
         {
-            // row li of the history the next direction sees: the old row li, or li + 1 when
-            // the oldest was dropped, or the new pair
-            const bool full = hl_now >= m;
-            const bool is_new = acc_ok && li == (full ? m - 1 : hl_now);
-            const int src = min((acc_ok && full) ? li + 1 : li, KH - 1);
+            // chronological row li of the history the next direction sees, by ring slot
+            const int head_n = acc_ok && full ? ring(head, 1, m) : head;
+            const int ps = ring(head_n, li < m ? li : 0, m);
+            const bool is_new = acc_ok && ps == q;
             const float an_s = lane_f(rn, many_src<4>(0)), an_y = lane_f(rn, many_src<4>(1));
-            const float ao = __shfl(rf, many_src<KF>(src)), bo = __shfl(rf, many_src<KF>(KH + src));
+            const float ao = __shfl(rf, many_src<KF>(ps)), bo = __shfl(rf, many_src<KF>(KH + ps));
             a_l = is_new ? an_s : ao;
             b_l = is_new ? an_y : bo;
         }
@@ -1036,7 +1051,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         __syncthreads();
     }
     // ---- write back ----
-    const int hl_f = S.hist_len, head_f = S.head;
+    const int hl_f = S.hist_len;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int k = lane + WAVE * r;
@@ -1046,7 +1061,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
                 if (i < hl_f) {
-                    const long long o = (long long)ring(head_f, i, m) * n + k;
+                    const long long o = (long long)i * n + k;
                     Sh[o] = hs[i][r];
                     Yh[o] = hy[i][r];
                 }
