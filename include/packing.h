@@ -91,7 +91,8 @@ int pk_boundary(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst
  * with an explicit handle.  mirror = 1 reproduces the reference's half table (x >= 0
  * stored, x < 0 looked up as (-x, -theta); core/pack_cuda_lut.py:128-131,314-320);
  * mirror = 0 is a full-range table for polygons without mirror symmetry.
- * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).
+ * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).  At most 2^30 - 1
+ * nodes (4 GiB of float32; the kernels address the table with 32-bit byte offsets).
  */
 int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, double x_max,
                   double y_min, double y_max, double t_min, double t_max, int quad, int mirror,

@@ -63,13 +63,16 @@ __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, L
     // theta pair (it0, min(it0 + 1, nt - 1)) read as nodes (bt, bt + 1), bt <= nt - 2
     const int bt = min(it0, T_.nt - 2);
     t.top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
-    const float* __restrict__ v = T_.vals;
-    const size_t r00 = ((size_t)ix0 * T_.ny + iy0) * T_.nt + bt, r01 = ((size_t)ix0 * T_.ny + iy1) * T_.nt + bt;
-    const size_t r10 = ((size_t)ix1 * T_.ny + iy0) * T_.nt + bt, r11 = ((size_t)ix1 * T_.ny + iy1) * T_.nt + bt;
-    __builtin_memcpy(&t.p00, v + r00, 8);
-    __builtin_memcpy(&t.p01, v + r01, 8);
-    __builtin_memcpy(&t.p10, v + r10, 8);
-    __builtin_memcpy(&t.p11, v + r11, 8);
+    // 32-bit byte offsets from the table base (pk_lut_create keeps tables below 2^30
+    // nodes), so each gather is one base register plus one offset
+    const char* __restrict__ v = reinterpret_cast<const char*>(T_.vals);
+    const uint32_t ny = (uint32_t)T_.ny, nt = (uint32_t)T_.nt;
+    const uint32_t o00 = (((uint32_t)ix0 * ny + (uint32_t)iy0) * nt + (uint32_t)bt) * 4u;
+    const uint32_t oy = (uint32_t)(iy1 - iy0) * nt * 4u, ox = (uint32_t)(ix1 - ix0) * ny * nt * 4u;
+    __builtin_memcpy(&t.p00, v + o00, 8);
+    __builtin_memcpy(&t.p01, v + (o00 + oy), 8);
+    __builtin_memcpy(&t.p10, v + (o00 + ox), 8);
+    __builtin_memcpy(&t.p11, v + (o00 + ox + oy), 8);
 }
 
 // Trilinear value and analytic gradient w.r.t. (x, y, theta) of the table

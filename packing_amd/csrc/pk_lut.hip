@@ -117,6 +117,7 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
     PK_REQUIRE(out != nullptr && vals != nullptr, "pk_lut_create: NULL argument");
     PK_REQUIRE(nx >= 2 && ny >= 2 && nt >= 2, "pk_lut_create: every axis needs >= 2 nodes");
     PK_REQUIRE(x_max > x_min && y_max > y_min && t_max > t_min, "pk_lut_create: empty range");
+    PK_REQUIRE((long long)nx * ny * nt < (1ll << 30), "pk_lut_create: at most 2^30 - 1 nodes");
     Lut* L = new Lut();
     L->d.vals = vals;
     L->d.nx = nx;
