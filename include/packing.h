@@ -91,8 +91,12 @@ int pk_boundary(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst
  * with an explicit handle.  mirror = 1 reproduces the reference's half table (x >= 0
  * stored, x < 0 looked up as (-x, -theta); core/pack_cuda_lut.py:128-131,314-320);
  * mirror = 0 is a full-range table for polygons without mirror symmetry.
- * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).  At most 2^30 - 1
- * nodes (4 GiB of float32; the kernels address the table with 32-bit byte offsets).
+ * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).  At most 2^28 - 1
+ * nodes.  The handle keeps its own corner-quad copy of vals (4x the table's memory, on the
+ * table's device): entry (ix, iy, it) holds the four (x, y) corners of a cell at theta
+ * node it, so a lookup reads two adjacent 16-byte entries instead of four scattered
+ * pairs.  The copy is taken here (the call synchronizes that device), so writes to vals
+ * after pk_lut_create are not seen; create a new handle instead.
  */
 int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, double x_max,
                   double y_min, double y_max, double t_min, double t_max, int quad, int mirror,

@@ -29,7 +29,9 @@ enum { MODE_AREA = 0, MODE_SEP = 1, MODE_LUT = 2 };
 
 // Pose-space lookup table in device memory: vals[ix][iy][it], theta fastest.
 struct LutDev {
-    const float* vals;
+    const float* vals;     // vals[ix][iy][it], theta fastest (the caller's table)
+    const float4* corners; // corners[ix][iy][it] = values at (ix, iy), (ix, iy+1), (ix+1, iy),
+                           // (ix+1, iy+1) and node it, upper indices clamped (library's copy)
     int nx, ny, nt;
     float x_lo, x_min, x_max, y_min, y_max, t_min, t_max;   // x_lo = range check low end
     float sx, sy, st;                                       // (n - 1) / (max - min)
@@ -38,13 +40,13 @@ struct LutDev {
 };
 
 // One table lookup split in two so that the loads of several lookups can be in flight
-// together: lut_fetch computes the cell and issues the loads (the two theta-neighbour
-// corners of each (x, y) node come in one 8-byte load), lut_combine interpolates.
-// Gathered corners of one lookup as loaded: theta pairs (bt, bt + 1) at the four (x, y)
-// corners; top selects node bt + 1 for both theta corners.  The selection is left to
-// lut_combine so that nothing waits on the loads before the interpolation.
+// together: lut_fetch computes the cell and issues the loads, lut_combine interpolates.
+// The eight corners of a cell are the four (x, y) corners of the corner copy at theta nodes
+// bt and bt + 1: two adjacent 16-byte entries, one cache line in most cases (the caller's
+// table would need four lines).  top selects node bt + 1 for both theta corners; the
+// selection is left to lut_combine so that nothing waits on the loads before it.
 struct LutTap {
-    float2 p00, p01, p10, p11;
+    float4 e0, e1;
     float fx, fy, ft;
     bool flip, in, top;
 };
@@ -63,16 +65,14 @@ __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, L
     // theta pair (it0, min(it0 + 1, nt - 1)) read as nodes (bt, bt + 1), bt <= nt - 2
     const int bt = min(it0, T_.nt - 2);
     t.top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
-    // 32-bit byte offsets from the table base (pk_lut_create keeps tables below 2^30
-    // nodes), so each gather is one base register plus one offset
-    const char* __restrict__ v = reinterpret_cast<const char*>(T_.vals);
-    const uint32_t ny = (uint32_t)T_.ny, nt = (uint32_t)T_.nt;
-    const uint32_t o00 = (((uint32_t)ix0 * ny + (uint32_t)iy0) * nt + (uint32_t)bt) * 4u;
-    const uint32_t oy = (uint32_t)(iy1 - iy0) * nt * 4u, ox = (uint32_t)(ix1 - ix0) * ny * nt * 4u;
-    __builtin_memcpy(&t.p00, v + o00, 8);
-    __builtin_memcpy(&t.p01, v + (o00 + oy), 8);
-    __builtin_memcpy(&t.p10, v + (o00 + ox), 8);
-    __builtin_memcpy(&t.p11, v + (o00 + ox + oy), 8);
+    (void)ix1;
+    (void)iy1;   // the corner copy holds the clamped upper corners
+    // 32-bit byte offset from the base of the corner copy (pk_lut_create keeps tables below
+    // 2^28 nodes): one base register plus one offset
+    const char* __restrict__ v = reinterpret_cast<const char*>(T_.corners);
+    const uint32_t o = (((uint32_t)ix0 * (uint32_t)T_.ny + (uint32_t)iy0) * (uint32_t)T_.nt + (uint32_t)bt) * 16u;
+    __builtin_memcpy(&t.e0, v + o, 16);
+    __builtin_memcpy(&t.e1, v + (o + 16u), 16);
 }
 
 // Trilinear value and analytic gradient w.r.t. (x, y, theta) of the table
@@ -81,10 +81,10 @@ __device__ inline float lut_combine(const LutDev& T_, const LutTap& t, float (&g
     g[0] = g[1] = g[2] = 0.f;
     if (!t.in) return 0.f;
     const float fx = t.fx, fy = t.fy, ft = t.ft;
-    const float v000 = t.top ? t.p00.y : t.p00.x, v001 = t.p00.y;
-    const float v010 = t.top ? t.p01.y : t.p01.x, v011 = t.p01.y;
-    const float v100 = t.top ? t.p10.y : t.p10.x, v101 = t.p10.y;
-    const float v110 = t.top ? t.p11.y : t.p11.x, v111 = t.p11.y;
+    const float v000 = t.top ? t.e1.x : t.e0.x, v001 = t.e1.x;
+    const float v010 = t.top ? t.e1.y : t.e0.y, v011 = t.e1.y;
+    const float v100 = t.top ? t.e1.z : t.e0.z, v101 = t.e1.z;
+    const float v110 = t.top ? t.e1.w : t.e0.w, v111 = t.e1.w;
     const float v00 = v000 * (1.f - ft) + v001 * ft, v01 = v010 * (1.f - ft) + v011 * ft;
     const float v10 = v100 * (1.f - ft) + v101 * ft, v11 = v110 * (1.f - ft) + v111 * ft;
     const float v0 = v00 * (1.f - fy) + v01 * fy, v1 = v10 * (1.f - fy) + v11 * fy;

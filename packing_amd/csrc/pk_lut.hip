@@ -1,5 +1,6 @@
 // Pose-space lookup table: handle, on-device builder (exact clipping of the polygon at the
 // origin against the polygon at every grid pose) and the table-based overlap cost.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -10,7 +11,30 @@ namespace pk {
 
 struct Lut {
     LutDev d;
+    float4* corners = nullptr;   // owned corner copy of the caller's table
+    int device = 0;
 };
+
+// Corner copy of a table: entry (ix, iy, it) holds the values at (ix, iy), (ix, iy+1),
+// (ix+1, iy), (ix+1, iy+1) and theta node it, upper indices clamped to the last node, so
+// the eight corners of a cell are two adjacent entries.
+__global__ __launch_bounds__(256) void k_lut_corners(const float* __restrict__ vals, int nx, int ny, int nt,
+                                                     float4* __restrict__ quad) {
+    const long long total = (long long)nx * ny * nt;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int it = (int)(q % nt);
+        const long long r = q / nt;
+        const int iy = (int)(r % ny), ix = (int)(r / ny);
+        const int ix1 = min(ix + 1, nx - 1), iy1 = min(iy + 1, ny - 1);
+        float4 e;
+        e.x = vals[((long long)ix * ny + iy) * nt + it];
+        e.y = vals[((long long)ix * ny + iy1) * nt + it];
+        e.z = vals[((long long)ix1 * ny + iy) * nt + it];
+        e.w = vals[((long long)ix1 * ny + iy1) * nt + it];
+        quad[q] = e;
+    }
+}
 
 LutDev make_lut_dev(const void* h) { return reinterpret_cast<const Lut*>(h)->d; }
 
@@ -117,9 +141,35 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
     PK_REQUIRE(out != nullptr && vals != nullptr, "pk_lut_create: NULL argument");
     PK_REQUIRE(nx >= 2 && ny >= 2 && nt >= 2, "pk_lut_create: every axis needs >= 2 nodes");
     PK_REQUIRE(x_max > x_min && y_max > y_min && t_max > t_min, "pk_lut_create: empty range");
-    PK_REQUIRE((long long)nx * ny * nt < (1ll << 30), "pk_lut_create: at most 2^30 - 1 nodes");
+    const long long nodes = (long long)nx * ny * nt;
+    PK_REQUIRE(nodes < (1ll << 28), "pk_lut_create: at most 2^28 - 1 nodes");
+    // the copy lives on the table's device; the caller's writes to vals are complete
+    // (whatever stream made them) before it is taken
+    hipPointerAttribute_t attr;
+    PK_CHECK_HIP(hipPointerGetAttributes(&attr, vals));
+    int prev = 0;
+    PK_CHECK_HIP(hipGetDevice(&prev));
+    PK_CHECK_HIP(hipSetDevice(attr.device));
+    float4* cq = nullptr;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMalloc((void**)&cq, sizeof(float4) * (size_t)nodes);
+    if (e == hipSuccess) {
+        const int blocks = (int)std::min<long long>((nodes + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_lut_corners, dim3(blocks), dim3(256), 0, 0, vals, nx, ny, nt, cq);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) {
+        if (cq) (void)hipFree(cq);
+        set_error(std::string("pk_lut_create: ") + hipGetErrorString(e));
+        return PK_EHIP;
+    }
     Lut* L = new Lut();
+    L->corners = cq;
+    L->device = attr.device;
     L->d.vals = vals;
+    L->d.corners = cq;
     L->d.nx = nx;
     L->d.ny = ny;
     L->d.nt = nt;
@@ -140,7 +190,15 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
 }
 
 int pk_lut_destroy(pk_lut_t h) {
-    delete reinterpret_cast<Lut*>(h);
+    Lut* L = reinterpret_cast<Lut*>(h);
+    if (L && L->corners) {
+        int prev = 0;
+        if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(L->device) == hipSuccess) {
+            (void)hipFree(L->corners);
+            (void)hipSetDevice(prev);
+        }
+    }
+    delete L;
     return PK_OK;
 }
 
