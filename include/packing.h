@@ -131,7 +131,9 @@ int pk_lut_overlap(pk_geometry_t g, pk_lut_t lut, const float* xyt, const int32_
  * evaluates the cost and starts the run).  One call advances every listed slot by up to
  * `iters` iterations; a slot stops for good when it converges or reaches max_iter, with
  * the reference's tests applied to it alone (batch-independent).  Workgroup b runs slot
- * order[b] (order NULL: b).  status_out[s] = 1 once slot s is finished.
+ * order[b] (order NULL: b).  status_out[s] = 1 once slot s is finished.  History row r of a
+ * slot is ring slot r (the record holds the count and the oldest slot), the layout every
+ * kernel choice reads and writes.
  */
 typedef struct {
     int n_slots;
@@ -153,7 +155,11 @@ typedef struct {
     float lr, max_step, tol_grad, tol_change, tol_rel;   /* max_step <= 0: no clamp  */
     int stop_on_increase;
     int iters;                 /* iterations this call                                */
-    int block;                 /* 64, 128, 256 (0 = 256)                              */
+    int block;                 /* 0: library's choice (1 when N <= 212 and history
+                                  <= 8, else 256); 1: one wave per instance, state
+                                  and history in registers (PK_ERANGE beyond those
+                                  limits); 64, 128, 256: threads per instance.  Every
+                                  choice gives the same bits.                        */
     int32_t* status_out;       /* [n_slots] or NULL                                   */
     float* loss_out;           /* [n_slots] or NULL                                   */
     unsigned long long* stats; /* [2] tree pairs tested-in, pairs clipped/looked up   */
@@ -165,7 +171,8 @@ typedef struct {
 int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
 
 /* Dynamic LDS bytes per workgroup of pk_relax_step for the given mode, largest tree count,
- * history length and block (64, 128, 256), with the history in LDS (hist_lds 1) or not (0). */
+ * history length and block (1, 64, 128, 256), with the history in LDS (hist_lds 1) or not
+ * (0; ignored for block 1, whose history is in registers). */
 int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max, int history, int hist_lds,
                        int block);
 
