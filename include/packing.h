@@ -66,7 +66,8 @@ int pk_geometry_destroy(pk_geometry_t g);
  * (cost only).  crystal (n_ens, 4) = lattice axes [ax, ay, bx, by] or NULL; with it the
  * sum runs over 0 <= |cell| <= n_periodic images, and only_self keeps only a tree's own
  * images (core/pack_cuda.py:332-364).  block = 64, 128 or 256 threads per instance
- * (0 = 256).  stats (2 uint64, device, accumulated) = [tree pairs within the cutoff,
+ * (0 = the library's choice: 64 up to 150 trees, else 128; float64 always runs 256); the
+ * result bits do not depend on it.  stats (2 uint64, device, accumulated) = [tree pairs within the cutoff,
  * piece pairs clipped], or NULL.
  */
 int pk_overlap(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst_off, int n_ens,

@@ -302,6 +302,10 @@ int pk_overlap(pk_geometry_t h, int dtype, const void* xyt, const int32_t* inst_
     PK_REQUIRE(xyt && out_cost, "pk_overlap: xyt and out_cost are required");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (dtype == PK_F32) {
+        // library's choice: one wave per instance up to 150 trees, two above (measured on
+        // MI355X at 50, 100 and 200 trees; 256 threads never won); results are the same
+        // bits for every block
+        if (block == 0) block = n_trees <= 150 ? 64 : 128;
         switch (block) {
             case 64:
                 return dispatch_overlap<float, 64>(g, xyt, inst_off, n_ens, n_trees, mode, crystal,
@@ -309,7 +313,6 @@ int pk_overlap(pk_geometry_t h, int dtype, const void* xyt, const int32_t* inst_
             case 128:
                 return dispatch_overlap<float, 128>(g, xyt, inst_off, n_ens, n_trees, mode, crystal,
                                                     n_periodic, only_self, out_cost, out_grads, stats, st);
-            case 0:
             case 256:
                 return dispatch_overlap<float, 256>(g, xyt, inst_off, n_ens, n_trees, mode, crystal,
                                                     n_periodic, only_self, out_cost, out_grads, stats, st);

@@ -718,10 +718,11 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
 // budget of 256 then holds the state, the history and the evaluator), one above.
 template <int R>
 constexpr int wave_occupancy() { return R <= 5 ? 2 : 1; }
-// Table lookups per lane in flight at once: four where one wave has the SIMD to itself,
-// two where a second wave hides the gather latency and registers are scarcer.
+// Table lookups per lane in flight at once: four for the largest instances (R >= 9, one
+// wave per SIMD whatever the choice), two below, where the registers saved let a wave of
+// a small-instance launch share the SIMD (R <= 8 with R = 3 fits the 512 registers).
 template <int R>
-constexpr int wave_lookups() { return wave_occupancy<R>() == 1 ? 4 : 2; }
+constexpr int wave_lookups() { return R >= 9 ? 4 : 2; }
 
 // Evaluation of the register state x into g for k_relax_w (through the LDS vectors xs, gs).
 template <int MODE, int R>
