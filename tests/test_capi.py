@@ -27,3 +27,13 @@ def test_binding_covers_header():
 
 def test_version():
     assert _native.lib().pk_version() == 1
+
+
+def test_lut_create_rejects_oversized_table():
+    """Tables of 2^28 nodes or more are refused before any device work (no GPU needed)."""
+    import ctypes as C
+    L = _native.lib()
+    h = C.c_void_p()
+    rc = L.pk_lut_create(C.c_void_p(16), 1024, 1024, 256, -1.0, 1.0, -1.0, 1.0, -3.0, 3.0, 0, 0, C.byref(h))
+    assert rc != 0
+    assert b"2^28" in L.pk_last_error()
