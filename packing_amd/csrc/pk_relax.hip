@@ -6,13 +6,15 @@
 // gradient w.r.t. h zeroed when h is fixed (core/pack_cost.py:178-188).
 // Optimizer: core/lbfgs_torch_parallel.py:411-729 without line search, applied to each
 // instance on its own (a batch of one), so an instance's result never depends on what
-// else is in the launch: two-loop recursion, first step lr*min(1, 1/sum|g|), max-step
-// clamp, directional-derivative stop, history append/shift only from iteration 2 with
-// y.s > 1e-10, and the reference's convergence tests.
+// else is in the launch: the two-loop direction (computed in the equivalent compact
+// form), first step lr*min(1, 1/sum|g|), max-step clamp, directional-derivative stop,
+// history append (a ring of slots) only from iteration 2 with y.s > 1e-10, and the
+// reference's convergence tests.
 //
-// All vector reductions run on wave 0 in a fixed order (lane-strided partial sums in
-// double, then a fixed DPP / lane-read tree), so results are independent of the
-// workgroup size.  The history lives in LDS for the duration of a call when it fits.
+// Two kernels: k_relax, one workgroup per instance with the vectors (and the history
+// when it fits) in LDS and the L-BFGS arithmetic on wave 0; k_relax_w, one wave per
+// instance with the state and history in registers.  Every reduction has a fixed order,
+// so both give the same bits for any workgroup size and may continue each other's runs.
 #include <cstring>
 #include <string>
 
@@ -195,10 +197,12 @@ struct Hist {
 // L-BFGS arithmetic shared by both relaxation kernels.  Vectors are walked element by
 // element, k = lane + 64 r with r increasing; per-lane partial sums are float fused
 // multiply-adds (at most a few terms per lane), and every wave-wide sum adds the 64
-// partials in double through wave_sum_many, whose result does not depend on the list it
+// partials through the butterflies of pk_common.h (in float for history dot products, in
+// double for the step quantities), whose result does not depend on the list a value
 // sits in.  Both kernels therefore produce the same bits, wherever the vectors live.
-// History rows are passed in chronological order (oldest first); rows at or beyond the
-// used count are ignored.
+// Dot products with history rows are separate sums, so the order the rows come in
+// (chronological in k_relax, by ring slot in k_relax_w) does not change them; the
+// direction adds its history terms in ring-slot order in both kernels.
 // ---------------------------------------------------------------------------------
 
 constexpr int pow2_ceil(int k) { return k <= 1 ? 1 : 2 * pow2_ceil((k + 1) / 2); }
@@ -388,7 +392,8 @@ struct Lb {
     }
 };
 
-// History rows of element k for the first nv rows (chronological), from LDS or global memory.
+// History rows of element k for the first nv entries of off (row offsets), from LDS or
+// global memory; entries at or beyond nv read as zero.
 template <int KH, bool HL>
 __device__ inline void hist_elem(const Hist<HL>& H, const int (&off)[KH], int k, int nv, float (&sv)[KH],
                                  float (&yv)[KH]) {
@@ -748,8 +753,8 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
 
 // One wave per instance, with the state vectors (x, g, g_prev, d) and the L-BFGS history
 // in registers: element k of a state vector is row k / 64 of lane k % 64 (R rows cover
-// 64 R >= 3N + 3 elements; rows beyond the state are zero), history rows in chronological
-// order (oldest first, shifted down when the ring is full).  Only the evaluator's tables
+// 64 R >= 3N + 3 elements; rows beyond the state are zero), history row i = ring slot i
+// (a new pair overwrites one slot; unused slots are zero).  Only the evaluator's tables
 // and the small matrices live in LDS, so many instances are resident per CU and no
 // workgroup barrier ever waits on idle waves.  The arithmetic is that of k_relax (the
 // same helpers, the same per-lane order), so both kernels give the same bits and may
