@@ -51,7 +51,7 @@ struct GeomT {
     T cutoff_sq;                  // (2 radius)^2
     T px[MAXP * MAXV], py[MAXP * MAXV];   // local piece vertices
     T ccx[MAXP], ccy[MAXP], cr[MAXP];     // local piece bounding circles
-    T ox[MAXO], oy[MAXO];                 // outline vertices
+    T ox[MAXO], oy[MAXO];                 // outline vertices, padded to MAXO with the last
 };
 
 // Host handle: the same polygon in both precisions.

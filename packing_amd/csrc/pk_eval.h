@@ -991,37 +991,42 @@ __device__ inline bool boundary_inside(const GeomT<T>& G, T x, T y, T half, T hx
 
 // Boundary term of one tree (core/pack_cuda.py:550-642): max over outline vertices of
 // the squared distance outside the square h = [size, ox, oy]; gradients w.r.t. the pose
-// and h at the arg-max vertex (strict >, first vertex on ties, 0 if inside).
-template <typename T>
+// and h at the arg-max vertex (strict >, first vertex on ties, vertex 0 and zero if
+// inside).  The outline is padded to MAXO with its last vertex, so whole blocks of 8 are
+// read (one LDS round trip each); the arg-max vertex is carried, not read again.  CHECK:
+// test the bounding circle first (callers that already know the tree is near the
+// boundary skip it).
+template <typename T, bool CHECK = true>
 __device__ inline T boundary_tree(const GeomT<T>& G, T x, T y, T c, T s, T half, T hx, T hy,
                                   T (&g)[3], T (&gh)[3]) {
-    if (boundary_inside(G, x, y, half, hx, hy)) {
+    if (CHECK && boundary_inside(G, x, y, half, hx, hy)) {
         g[0] = g[1] = g[2] = T(0);
         gh[0] = gh[1] = gh[2] = T(0);
         return T(0);
     }
     T best = T(0);
-    int bi = 0;
+    T lx = G.ox[0], ly = G.oy[0];
     const int no = G.n_outline;
-    for (int v0 = 0; v0 < no; v0 += 8) {   // blocks of 8 vertices, loads issued together
+    for (int v0 = 0; v0 < no; v0 += 8) {
         T lxs[8], lys[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int v = min(v0 + u, no - 1);
-            lxs[u] = G.ox[v];
-            lys[u] = G.oy[v];
+            lxs[u] = G.ox[v0 + u];
+            lys[u] = G.oy[v0 + u];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const T lx = lxs[u], ly = lys[u];
-            const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
+            const T vx = c * lxs[u] - s * lys[u] + x - hx, vy = s * lxs[u] + c * lys[u] + y - hy;
             const T ax = fabs(vx), ay = fabs(vy);
             const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);
             const T d2 = dx * dx + dy * dy;
-            if (v0 + u < no && d2 > best) { best = d2; bi = v0 + u; }
+            if (d2 > best) {
+                best = d2;
+                lx = lxs[u];
+                ly = lys[u];
+            }
         }
     }
-    const T lx = G.ox[bi], ly = G.oy[bi];
     const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
     const T ax = fabs(vx), ay = fabs(vy);
     const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);

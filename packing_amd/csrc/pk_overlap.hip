@@ -256,11 +256,14 @@ int pk_geometry_create(const double* piece_xy, const int32_t* piece_nverts, int 
         g->hd.ccx[p] = cx; g->hd.ccy[p] = cy; g->hd.cr[p] = r;
         g->hf.ccx[p] = (float)cx; g->hf.ccy[p] = (float)cy; g->hf.cr[p] = (float)r;
     }
-    for (int v = 0; v < n_outline; ++v) {
-        g->hd.ox[v] = outline_xy[2 * v];
-        g->hd.oy[v] = outline_xy[2 * v + 1];
-        g->hf.ox[v] = (float)outline_xy[2 * v];
-        g->hf.oy[v] = (float)outline_xy[2 * v + 1];
+    // the outline padded to MAXO by repeating its last vertex (boundary_tree reads whole
+    // blocks of 8; a repeat never wins its strict arg-max)
+    for (int v = 0; v < MAXO; ++v) {
+        const int w = v < n_outline ? v : n_outline - 1;
+        g->hd.ox[v] = outline_xy[2 * w];
+        g->hd.oy[v] = outline_xy[2 * w + 1];
+        g->hf.ox[v] = (float)outline_xy[2 * w];
+        g->hf.oy[v] = (float)outline_xy[2 * w + 1];
     }
     hipError_t e1 = hipGetDevice(&g->device);
     hipError_t e2 = hipMalloc((void**)&g->df, sizeof(GeomT<float>));

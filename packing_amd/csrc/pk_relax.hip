@@ -114,7 +114,7 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
         int bn = 0;
         auto finish = [&](int i) {
             float gb[3], gh[3];
-            const float cb = boundary_tree<float>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
+            const float cb = boundary_tree<float, false>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
             lb[0] += Acc<float>::local(cb);
             lb[1] += Acc<float>::local(gh[0]);
             lb[2] += Acc<float>::local(gh[1]);
