@@ -126,8 +126,14 @@ def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0):
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` recorded from rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     (calibrated by a plain copy) in the committed profile summaries, or None."""
+    return profiled_value("traffic_bytes_per_launch", kernel)
+
+
+def profiled_value(name, kernel):
+    """A figure `name[kernel] = value` recorded in the committed profile summaries, with the
+    file it came from, or (None, None)."""
     import glob
-    key = f"traffic_bytes_per_launch[{kernel}]"
+    key = f"{name}[{kernel}]"
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         with open(f) as fh:
             for line in fh:
@@ -188,9 +194,13 @@ def bucket_roofline(b, sizes, opts, chunk, n_chunks, stream):
     alg = dom["bytes"] / n_chunks
     achieved = alg / avg_s / 1e9
     traffic, src = profiled_traffic(dom["kernel"])
+    issue, _ = profiled_value("issue_fraction", dom["kernel"])
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 5), "traffic": traffic,
             "traffic_source": src,
+            # share of the wave's cycles spent issuing instructions (one wave per SIMD):
+            # what actually bounds this kernel, from the same profile
+            "issue_frac": issue,
             "kernel": dom["kernel"] + f" (table mode, size bucket of {dom['instances']} instances, "
                                       f"N <= {dom['n_max']})",
             "avg_launch_ms": round(avg_s * 1e3, 3), "alg_bytes_per_launch": alg,
