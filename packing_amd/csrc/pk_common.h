@@ -82,6 +82,7 @@ struct Acc<float> {
     __device__ static inline void add_raw(type* p, type v) { atomicAdd(p, v); }
     __device__ static inline float get(type v) { return (float)((double)(long long)v * FIX_INV); }
     __device__ static inline type zero() { return 0ull; }
+    __device__ static inline type neg(type v) { return 0ull - v; }   // local(-x) = neg(local(x))
 };
 
 template <>
@@ -92,6 +93,7 @@ struct Acc<double> {
     __device__ static inline void add_raw(type* p, type v) { atomicAdd(p, v); }
     __device__ static inline double get(type v) { return v; }
     __device__ static inline type zero() { return 0.0; }
+    __device__ static inline type neg(type v) { return -v; }
 };
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) / 16 * 16; }

@@ -598,11 +598,14 @@ struct Evaluator {
         if (GRAD) {
             const int i = q.i, j = q.j;
             const float cr = q.cr, sr = q.sr;
-            AccT::add(&L.g[3 * i + 0], -gl[0] * cr + gl[1] * sr);
-            AccT::add(&L.g[3 * i + 1], -gl[0] * sr - gl[1] * cr);
+            // the centre gradients of the pair are exact opposites: tree i's are the
+            // negated fixed-point values of tree j's (the conversion rounds symmetrically)
+            const A gx = AccT::local(gl[0] * cr - gl[1] * sr), gy = AccT::local(gl[0] * sr + gl[1] * cr);
+            AccT::add_raw(&L.g[3 * i + 0], AccT::neg(gx));
+            AccT::add_raw(&L.g[3 * i + 1], AccT::neg(gy));
             AccT::add(&L.g[3 * i + 2], gl[0] * q.dyl - gl[1] * q.dxl - gl[2]);
-            AccT::add(&L.g[3 * j + 0], gl[0] * cr - gl[1] * sr);
-            AccT::add(&L.g[3 * j + 1], gl[0] * sr + gl[1] * cr);
+            AccT::add_raw(&L.g[3 * j + 0], gx);
+            AccT::add_raw(&L.g[3 * j + 1], gy);
             AccT::add(&L.g[3 * j + 2], gl[2]);
         }
     }
