@@ -146,7 +146,13 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
     // the copy lives on the table's device; the caller's writes to vals are complete
     // (whatever stream made them) before it is taken
     hipPointerAttribute_t attr;
-    PK_CHECK_HIP(hipPointerGetAttributes(&attr, vals));
+    if (hipPointerGetAttributes(&attr, vals) != hipSuccess) {
+        (void)hipGetLastError();   // not a device allocation: clear the runtime's error
+        set_error("pk_lut_create: vals must be device memory");
+        return PK_EINVAL;
+    }
+    PK_REQUIRE(attr.type == hipMemoryTypeDevice || attr.isManaged,
+               "pk_lut_create: vals must be device memory");
     int prev = 0;
     PK_CHECK_HIP(hipGetDevice(&prev));
     PK_CHECK_HIP(hipSetDevice(attr.device));
