@@ -46,6 +46,32 @@ def test_first_iterations_vs_oracle(oracle_mod, gname, mode, iters):
         assert np.abs(got - ref).max() < 2e-5, (i, np.abs(got - ref).max())
 
 
+@pytest.mark.parametrize("gname,full_x", [("tree", False), ("random", True)])
+@pytest.mark.parametrize("iters", [1, 2, 4])
+def test_first_iterations_vs_oracle_table(oracle_mod, gname, full_x, iters):
+    """Table mode (the benchmark's cost): early iterates agree with the FP64 oracle's
+    L-BFGS over the same float32 table (mirrored half table for the symmetric tree)."""
+    from packing_amd.context import set_geometry
+    from packing_amd.pack_cuda_lut import LookupTable
+    geom = GEOMS[gname]()
+    set_geometry(geom)
+    lut = LookupTable.build_exact(96, 96, 64, full_x=full_x)
+    OL = oracle_mod.OracleLut(lut.X, lut.Y, lut.theta, lut.lut_d.cpu().numpy(), mirror=lut.mirror_x)
+    sizes = [5, 12, 30, 41]
+    xs, hs = ragged(3, sizes)
+    b = batch(geom, xs, hs, mode="lut", lut=lut, max_iter=iters, history=8, max_step=0.1)
+    b.run()
+    torch.cuda.synchronize()
+    og = oracle_mod.OracleGeometry(geom)
+    for i, n in enumerate(sizes):
+        ref = oracle_mod.relax(og, xs[i][None].astype(np.float64), hs[i][None], max_iter=iters, m=8,
+                               max_step=0.1, lut=OL)[0]
+        got = b.xyt(i).cpu().numpy()
+        assert np.abs(got - ref).max() < 2e-5, (i, np.abs(got - ref).max())
+        if iters == 4:
+            assert np.abs(got - xs[i]).max() > 1e-3   # the poses did move
+
+
 def test_invariances():
     """Alone vs in a batch, plain vs batched, chunked vs one launch, any block size,
     graph replay: bitwise the same poses."""
