@@ -14,13 +14,16 @@ def _gpu():
     require_gpu()
 
 
-def test_served_equals_alone():
+@pytest.mark.parametrize("mode", ["area", "lut"])
+def test_served_equals_alone(mode):
     from packing_amd.context import set_geometry
+    from packing_amd.pack_cuda_lut import LookupTable
     from packing_amd.relax import RelaxBatch, RelaxOptions
     from packing_amd.serving import RelaxServer, poisson_requests
     geom = GEOMS["random"]()
     set_geometry(geom)
-    opts = RelaxOptions(mode="area", max_iter=60, history=8, max_step=0.1)
+    kw = {"lut": LookupTable.build_exact(96, 96, 64, full_x=True)} if mode == "lut" else {}
+    opts = RelaxOptions(mode=mode, max_iter=60, history=8, max_step=0.1, **kw)
     reqs = poisson_requests(40, rate=2000.0, n_lo=10, n_hi=60)
     srv = RelaxServer(opts, n_slots=16, n_max=60, chunk=20)
     rep = srv.serve(reqs)
