@@ -162,6 +162,29 @@ __device__ inline float lane_f(float v, int l) {
 
 constexpr int ilog2c(int k) { return k <= 1 ? 0 : 1 + ilog2c(k / 2); }
 
+// Butterfly partner exchange for lane bit `bit` (a compile-time constant after unrolling):
+// within a row of 16 lanes by DPP (quad permutes for bits 0 and 1, the half-row and row
+// mirrors for bits 2 and 3: the partner differs in that bit and in lower bits only, which
+// later stages handle), across rows by a lane permute.  Every value is then reduced by
+// one fixed pairing tree.  The full wave must be active.
+template <int CTRL>
+__device__ inline int dpp_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+__device__ inline int xchg_i(int v, int bit) {
+    switch (bit) {
+        case 1: return dpp_i<0xB1>(v);    // quad_perm [1,0,3,2]
+        case 2: return dpp_i<0x4E>(v);    // quad_perm [2,3,0,1]
+        case 4: return dpp_i<0x141>(v);   // row_half_mirror
+        case 8: return dpp_i<0x140>(v);   // row_mirror
+        default: return __shfl_xor(v, bit);
+    }
+}
+__device__ inline float xchg_f(float v, int bit) { return __int_as_float(xchg_i(__float_as_int(v), bit)); }
+__device__ inline double xchg_d(double v, int bit) {
+    return __hiloint2double(xchg_i(__double2hiint(v), bit), xchg_i(__double2loint(v), bit));
+}
+
 // (send, keep) = up ? (lo, hi) : (hi, lo), as a bitwise blend: a select between two
 // elements of a register array can be rewritten by the compiler into a dynamically
 // indexed load, which moves the whole array to scratch memory.
@@ -173,11 +196,12 @@ __device__ inline void blend_pair(double lo, double hi, bool up, double& send, d
 }
 
 // Sums of K values (K a power of two, 2..64) over the 64 lanes of a wave in one
-// butterfly.  At stage s the lanes l and l ^ (32 >> s) exchange halves of their value
-// lists and add, so after log2 K stages lane l holds value l >> (6 - log2 K); the
-// remaining stages add it across the other lanes.  Every value is summed by the same
-// pairing tree (pairs differing in lane bit 5, then 4, ..., 0), and each pair sum is
-// symmetric, so a value's total does not depend on K or on its position in the list.
+// butterfly.  At stage s the lanes l and its partner for bit b = 32 >> s (xchg_i: same
+// higher bits, opposite bit b) exchange halves of their value lists and add, so after
+// log2 K stages lane l holds value l >> (6 - log2 K); the remaining stages add it across
+// the other lanes.  Every value is summed by the same pairing tree (partners for bit 5,
+// then 4, ..., 0), and each pair sum is symmetric, so a value's total does not depend on
+// K or on its position in the list.
 // The full wave must be active.  Returns this lane's value.
 template <int K>
 __device__ inline double wave_sum_many(double (&v)[K]) {
@@ -193,12 +217,12 @@ __device__ inline double wave_sum_many(double (&v)[K]) {
         for (int j = 0; j < h; ++j) {
             double send, keep;
             blend_pair(v[j], v[h + j], up, send, keep);
-            v[j] = keep + shfl_xor_d(send, bit);
+            v[j] = keep + xchg_d(send, bit);
         }
     }
     double r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r += shfl_xor_d(r, 32 >> s);
+    for (int s = LK; s < 6; ++s) r += xchg_d(r, 32 >> s);
     return r;
 }
 // The same for float partial sums, added in double.
@@ -230,12 +254,12 @@ __device__ inline float wave_sum_many_f(float (&v)[K]) {
         for (int j = 0; j < h; ++j) {
             float send, keep;
             blend_pair_f(v[j], v[h + j], up, send, keep);
-            v[j] = keep + __shfl_xor(send, bit);
+            v[j] = keep + xchg_f(send, bit);
         }
     }
     float r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r += __shfl_xor(r, 32 >> s);
+    for (int s = LK; s < 6; ++s) r += xchg_f(r, 32 >> s);
     return r;
 }
 // The same layout with max instead of sum (exact in any order).
@@ -253,12 +277,12 @@ __device__ inline double wave_max_many(double (&v)[K]) {
         for (int j = 0; j < h; ++j) {
             double send, keep;
             blend_pair(v[j], v[h + j], up, send, keep);
-            v[j] = fmax(keep, shfl_xor_d(send, bit));
+            v[j] = fmax(keep, xchg_d(send, bit));
         }
     }
     double r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r = fmax(r, shfl_xor_d(r, 32 >> s));
+    for (int s = LK; s < 6; ++s) r = fmax(r, xchg_d(r, 32 >> s));
     return r;
 }
 // The same in float (max is exact, so the result equals the double version's).
@@ -276,12 +300,12 @@ __device__ inline float wave_max_many_f(float (&v)[K]) {
         for (int j = 0; j < h; ++j) {
             float send, keep;
             blend_pair_f(v[j], v[h + j], up, send, keep);
-            v[j] = fmaxf(keep, __shfl_xor(send, bit));
+            v[j] = fmaxf(keep, xchg_f(send, bit));
         }
     }
     float r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r = fmaxf(r, __shfl_xor(r, 32 >> s));
+    for (int s = LK; s < 6; ++s) r = fmaxf(r, xchg_f(r, 32 >> s));
     return r;
 }
 // Lane holding value j of a K-value butterfly result.
