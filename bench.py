@@ -27,12 +27,8 @@ def log(*a):
 
 
 def make_workload(n_inst, n_lo, n_hi, seed):
-    from packing_amd.geometry import box_for, make_poses
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(n_lo, n_hi + 1, size=n_inst)
-    xs = [make_poses(rng, 1, int(n), np.float32)[0] for n in sizes]
-    hs = np.stack([box_for(int(n)) for n in sizes]).astype(np.float32)
-    return sizes, xs, hs
+    from packing_amd.workloads import ragged_workload
+    return ragged_workload(n_inst, n_lo, n_hi, seed)
 
 
 # Arithmetic of the exact area kernel (add, sub, mul, reciprocal; a fused multiply-add

@@ -861,15 +861,19 @@ int or_lbfgs(or_fn f, void *ctx, double *x0, int M, int n, double lr, int max_it
 }
 
 /* ---------------------------------------------------------------------------
- * CPU relaxation baseline: cost = w_ov * overlap(area|sep) + w_bd * boundary, h fixed,
- * state x = [xyt | h] per instance as core/pack_dynamics.py:76-83 flattens it.
- * Each instance is its own L-BFGS run (M = 1), so the result does not depend on the
- * batch; instances run in parallel over OpenMP threads.
+ * CPU relaxation baseline: cost = w_ov * overlap(area|sep|table) + w_bd * boundary
+ * + w_ar * AreaCost, state x = [xyt | h] per instance as core/pack_dynamics.py:76-83
+ * flattens it.  With fixed_h the h gradient is zeroed and AreaCost is 0
+ * (core/pack_cost.py:178-188,1273-1284); otherwise h moves with the gradient
+ * w_bd * d(boundary)/dh + w_ar * (2 h0, 0, 0) (AreaCost = h0^2 for a square,
+ * core/kaggle_support.py:803-815; the collision term has no h gradient,
+ * core/pack_cost.py:624-629).  Each instance is its own L-BFGS run (M = 1), so the
+ * result does not depend on the batch; instances run in parallel over OpenMP threads.
  * ------------------------------------------------------------------------- */
 typedef struct {
     const or_geom *g;
-    int n_trees, use_sep;
-    double w_ov, w_bd;
+    int n_trees, use_sep, fixed_h;
+    double w_ov, w_bd, w_ar;
     const or_lut *lut;   /* non-NULL: table-based overlap term */
 } relax_ctx;
 
@@ -888,15 +892,26 @@ static void relax_fn(void *vc, const double *x, int M, int n, double *loss, doub
             or_overlap(c->g, xa, 1, N, c->use_sep, NULL, 0, 0, &co, ga);
         or_boundary(c->g, xa, xa + 3 * N, 1, N, &cb, gb, gh);
         for (k = 0; k < 3 * N; ++k) ga[k] = c->w_ov * ga[k] + c->w_bd * gb[k];
-        ga[3 * N] = ga[3 * N + 1] = ga[3 * N + 2] = 0.0;   /* fixed h */
         loss[a] = c->w_ov * co + c->w_bd * cb;
+        if (c->fixed_h) {
+            ga[3 * N] = ga[3 * N + 1] = ga[3 * N + 2] = 0.0;
+        } else {
+            double h0 = xa[3 * N];
+            ga[3 * N] = c->w_bd * gh[0] + c->w_ar * 2.0 * h0;
+            ga[3 * N + 1] = c->w_bd * gh[1];
+            ga[3 * N + 2] = c->w_bd * gh[2];
+            loss[a] += c->w_ar * h0 * h0;
+        }
     }
     free(gb); free(gh);
 }
 
-void or_relax(const or_geom *g, const or_lut *lut, double *xyt, const double *h, int n_ens,
-              int n_trees, int use_sep, double w_ov, double w_bd, int max_iter, int m,
-              double max_step, double lr, int n_threads) {
+/* xyt (n_ens, n_trees, 3) and h (n_ens, 3) are relaxed in place; out_iters (nullable)
+ * receives each instance's iteration count at exit (or_lbfgs's return value). */
+void or_relax(const or_geom *g, const or_lut *lut, double *xyt, double *h, int n_ens,
+              int n_trees, int use_sep, double w_ov, double w_bd, double w_ar, int fixed_h,
+              int max_iter, int m, double max_step, double lr, double tol_rel,
+              int stop_on_increase, int *out_iters, int n_threads) {
     int e;
     int n = 3 * n_trees + 3;
 #ifdef _OPENMP
@@ -904,12 +919,16 @@ void or_relax(const or_geom *g, const or_lut *lut, double *xyt, const double *h,
 #endif
     for (e = 0; e < n_ens; ++e) {
         relax_ctx c;
-        c.g = g; c.n_trees = n_trees; c.use_sep = use_sep; c.w_ov = w_ov; c.w_bd = w_bd; c.lut = lut;
+        c.g = g; c.n_trees = n_trees; c.use_sep = use_sep; c.w_ov = w_ov; c.w_bd = w_bd;
+        c.w_ar = w_ar; c.fixed_h = fixed_h; c.lut = lut;
         double *x = (double *)malloc(sizeof(double) * n);
         memcpy(x, xyt + (size_t)e * 3 * n_trees, sizeof(double) * 3 * n_trees);
         memcpy(x + 3 * n_trees, h + 3 * e, sizeof(double) * 3);
-        or_lbfgs(relax_fn, &c, x, 1, n, lr, max_iter, 0, 0.0, 0.0, 0.0, 0, m, max_step);
+        int it = or_lbfgs(relax_fn, &c, x, 1, n, lr, max_iter, 0, 0.0, 0.0, tol_rel, stop_on_increase,
+                          m, max_step);
+        if (out_iters) out_iters[e] = it;
         memcpy(xyt + (size_t)e * 3 * n_trees, x, sizeof(double) * 3 * n_trees);
+        memcpy(h + 3 * e, x + 3 * n_trees, sizeof(double) * 3);
         free(x);
     }
     (void)n_threads;

@@ -55,8 +55,8 @@ def lib():
         L.or_feasible.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, ip]
         L.or_relax.restype = None
         L.or_relax.argtypes = [C.c_void_p, C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_int,
-                               C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
-                               C.c_int]
+                               C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                               C.c_double, C.c_double, C.c_double, C.c_int, ip, C.c_int]
         FN = C.CFUNCTYPE(None, C.c_void_p, dp, C.c_int, C.c_int, dp, dp)
         L._FN = FN
         L.or_lbfgs.restype = C.c_int
@@ -204,16 +204,19 @@ def feasible_pairs(og: OracleGeometry, xyt):
 
 
 def relax(og: OracleGeometry, xyt, h, use_sep=False, w_ov=1.0, w_bd=1.0, max_iter=200, m=8,
-          max_step=0.5, lr=1.0, n_threads=1, lut: "OracleLut | None" = None):
-    """Per-instance L-BFGS relaxation in FP64 (returns relaxed xyt); lut selects the
-    table-based overlap term."""
+          max_step=0.5, lr=1.0, n_threads=1, lut: "OracleLut | None" = None, w_ar=0.0,
+          fixed_h=True, tol_rel=0.0, stop_on_increase=False, full=False):
+    """Per-instance L-BFGS relaxation in FP64; lut selects the table-based overlap term.
+    Returns the relaxed xyt, or (xyt, h, iterations) with full=True."""
     x = np.ascontiguousarray(xyt, dtype=np.float64).copy()
-    h = np.ascontiguousarray(h, dtype=np.float64)
+    hh = np.ascontiguousarray(h, dtype=np.float64).reshape(-1, 3).copy()
     E, N = x.shape[:2]
-    lib().or_relax(og.ptr, None if lut is None else C.byref(lut.s), _dp(x), _dp(h), E, N,
-                   int(use_sep), w_ov, w_bd, max_iter, m,
-                   max_step if max_step is not None else 0.0, lr, n_threads)
-    return x
+    its = np.zeros(E, dtype=np.int32)
+    lib().or_relax(og.ptr, None if lut is None else C.byref(lut.s), _dp(x), _dp(hh), E, N,
+                   int(use_sep), w_ov, w_bd, w_ar, int(fixed_h), max_iter, m,
+                   max_step if max_step is not None else 0.0, lr, tol_rel, int(stop_on_increase),
+                   its.ctypes.data_as(C.POINTER(C.c_int)), n_threads)
+    return (x, hh, its) if full else x
 
 
 def lbfgs(func, x0, lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-7,

@@ -1,10 +1,11 @@
 """The overlap cost as a torch.autograd.Function.
 
-Forward runs the cost-only kernel (no gradient accumulation); backward runs the
-cost+gradient kernel and scales each instance's gradient by the incoming grad_output.
-Both are the hand-written HIP kernels behind pk_overlap / pk_lut_overlap; the gradient is
-the analytic one (shape derivative of the clipped area, SAT depth derivative, or the
-trilinear table derivative).
+Forward runs the cost kernel once: with the gradient accumulated in the same launch when
+the poses require a gradient (the analytic gradient is saved for backward), cost only
+otherwise.  Backward scales each instance's saved gradient by the incoming grad_output;
+it launches nothing.  The kernels are the hand-written HIP kernels behind pk_overlap /
+pk_lut_overlap; the gradient is the analytic one (shape derivative of the clipped area,
+SAT depth derivative, or the trilinear table derivative).
 """
 from __future__ import annotations
 
@@ -18,24 +19,18 @@ class OverlapCost(torch.autograd.Function):
     def forward(ctx, xyt, use_separation=False, lut=None):
         xyt = xyt.contiguous()
         cost = torch.empty(xyt.shape[0], dtype=xyt.dtype, device=xyt.device)
+        g = torch.empty_like(xyt) if ctx.needs_input_grad[0] else None
         if lut is not None:
-            pack_cuda_lut.overlap_multi_ensemble(xyt, cost, lut)
+            pack_cuda_lut.overlap_multi_ensemble(xyt, cost, lut, out_grads=g)
         else:
-            pack_cuda.overlap_multi_ensemble(xyt, xyt, bool(use_separation), out_cost=cost)
-        ctx.save_for_backward(xyt)
-        ctx.use_separation = bool(use_separation)
-        ctx.lut = lut
+            pack_cuda.overlap_multi_ensemble(xyt, xyt, bool(use_separation), out_cost=cost, out_grads=g)
+        if g is not None:
+            ctx.save_for_backward(g)
         return cost
 
     @staticmethod
     def backward(ctx, grad_out):
-        (xyt,) = ctx.saved_tensors
-        cost = torch.empty(xyt.shape[0], dtype=xyt.dtype, device=xyt.device)
-        g = torch.empty_like(xyt)
-        if ctx.lut is not None:
-            pack_cuda_lut.overlap_multi_ensemble(xyt, cost, ctx.lut, out_grads=g)
-        else:
-            pack_cuda.overlap_multi_ensemble(xyt, xyt, ctx.use_separation, out_cost=cost, out_grads=g)
+        (g,) = ctx.saved_tensors
         return g * grad_out.reshape(-1, 1, 1).to(g.dtype), None, None
 
 

@@ -67,12 +67,12 @@ __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, L
     t.top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
     (void)ix1;
     (void)iy1;   // the corner copy holds the clamped upper corners
-    // 32-bit byte offset from the base of the corner copy (pk_lut_create keeps tables below
-    // 2^28 nodes): one base register plus one offset
-    const char* __restrict__ v = reinterpret_cast<const char*>(T_.corners);
-    const uint32_t o = (((uint32_t)ix0 * (uint32_t)T_.ny + (uint32_t)iy0) * (uint32_t)T_.nt + (uint32_t)bt) * 16u;
-    __builtin_memcpy(&t.e0, v + o, 16);
-    __builtin_memcpy(&t.e1, v + (o + 16u), 16);
+    // 32-bit entry index (pk_lut_create keeps tables below 2^32 nodes), scaled to a 64-bit
+    // address by one shift-add: tables of any size up to that, e.g. 900^3 nodes
+    const uint32_t idx = ((uint32_t)ix0 * (uint32_t)T_.ny + (uint32_t)iy0) * (uint32_t)T_.nt + (uint32_t)bt;
+    const float4* __restrict__ p = T_.corners + idx;
+    t.e0 = p[0];
+    t.e1 = p[1];
 }
 
 // Trilinear value and analytic gradient w.r.t. (x, y, theta) of the table

@@ -142,7 +142,11 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
     PK_REQUIRE(nx >= 2 && ny >= 2 && nt >= 2, "pk_lut_create: every axis needs >= 2 nodes");
     PK_REQUIRE(x_max > x_min && y_max > y_min && t_max > t_min, "pk_lut_create: empty range");
     const long long nodes = (long long)nx * ny * nt;
-    PK_REQUIRE(nodes < (1ll << 28), "pk_lut_create: at most 2^28 - 1 nodes");
+    PK_REQUIRE(nodes < (1ll << 32), "pk_lut_create: at most 2^32 - 1 nodes");
+    // a mirrored lookup indexes (-x, -theta): the theta range must be symmetric about 0, or a
+    // relative angle inside [t_min, t_max] could land below the first theta node
+    PK_REQUIRE(!mirror || (float)t_min == -(float)t_max,
+               "pk_lut_create: a mirrored table needs a theta range symmetric about 0");
     // the copy lives on the table's device; the caller's writes to vals are complete
     // (whatever stream made them) before it is taken
     hipPointerAttribute_t attr;

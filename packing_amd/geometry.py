@@ -119,6 +119,17 @@ class Geometry:
         if abs(tot - self.area) > 1e-9 * max(1.0, self.area):
             raise ValueError(f"pieces area {tot} != polygon area {self.area}")
 
+    @property
+    def mirror_symmetric(self) -> bool:
+        """True when the outline is its own mirror image about the local y axis, so the
+        overlap of two copies at relative pose (-x, y, -theta) equals that at (x, y, theta)
+        and a half table (x >= 0) describes every pose."""
+        v = np.asarray(self.outline, dtype=np.float64)
+        m = v * np.array([-1.0, 1.0])
+        tol = 1e-12 * max(1.0, float(np.abs(v).max()))
+        d = np.sqrt(((v[:, None, :] - m[None, :, :]) ** 2).sum(-1))
+        return bool(np.all(d.min(axis=1) <= tol))
+
     def fingerprint(self) -> str:
         import hashlib
         h = hashlib.md5()

@@ -125,8 +125,12 @@ class CollisionCost(Cost):
         if self.use_lookup_table and self._lut is None:
             if self.use_separation:
                 raise NotImplementedError("tables are built for the overlap area only")
+            # the reference tabulates the x >= 0 half (core/pack_cuda_lut.py:128-131), which
+            # holds for mirror-symmetric polygons only; others get the full x range
+            from .context import get_geometry
+            full_x = not get_geometry().mirror_symmetric
             self._lut = pack_cuda_lut.LookupTable.build_exact(self.lut_N_x, self.lut_N_y, self.lut_N_theta,
-                                                             full_x=True, trim_zeros=self.lut_trim_zeros)
+                                                             full_x=full_x, trim_zeros=self.lut_trim_zeros)
 
     def _internal(self, sol, cost, grad_xyt, evaluate_gradient, crystal_axes=None, only_self=False):
         pack_cuda.overlap_multi_ensemble(sol.xyt, sol.xyt, self.use_separation, out_cost=cost,

@@ -124,8 +124,8 @@ class LookupTable:
                                      Tt.data_ptr(), len(T), vals.data_ptr(),
                                      N.stream_handle(None)), "pk_lut_build")
         if trim_zeros:
-            Xn, Yn, Tn, vn = cls._trim_zero_edges(X, Y, T, vals.cpu().numpy(), verbose=verbose)
-            return cls(X=Xn, Y=Yn, theta=Tn, vals=vn, mirror_x=not full_x)
+            Xn, Yn, Tn, vn = cls._trim_zero_edges(X, Y, T, vals, verbose=verbose)
+            return cls(X=Xn, Y=Yn, theta=Tn, vals=vn.contiguous(), mirror_x=not full_x)
         return cls(X=X, Y=Y, theta=T, vals=vals, mirror_x=not full_x)
 
     @classmethod
@@ -158,9 +158,14 @@ class LookupTable:
                 return 0, len(mx) - 1
             return max(0, pos[0] - margin), min(len(mx) - 1, pos[-1] + margin)
 
-        x0, x1 = span(np.max(vals, axis=(1, 2)), 1)
-        y0, y1 = span(np.max(vals, axis=(0, 2)), 1)
-        t0, t1 = span(np.max(vals, axis=(0, 1)), 0)
+        if isinstance(vals, torch.Tensor):
+            # device tables are reduced where they live (the reference default is ~1.5 GB)
+            mx = [torch.amax(vals, dim=d).cpu().numpy() for d in ((1, 2), (0, 2), (0, 1))]
+        else:
+            mx = [np.max(vals, axis=d) for d in ((1, 2), (0, 2), (0, 1))]
+        x0, x1 = span(mx[0], 1)
+        y0, y1 = span(mx[1], 1)
+        t0, t1 = span(mx[2], 0)
         if verbose:
             print(f"trim: X {n_x}->{x1 - x0 + 1}, Y {n_y}->{y1 - y0 + 1}, theta {n_t}->{t1 - t0 + 1}")
         return X[x0:x1 + 1], Y[y0:y1 + 1], theta[t0:t1 + 1], vals[x0:x1 + 1, y0:y1 + 1, t0:t1 + 1]

@@ -263,6 +263,11 @@ class RelaxBatch:
         idx = np.concatenate([np.arange(o, o + 3 * n) for o, n in zip(self.xoff_np, self.sizes)])
         return self.x[torch.tensor(idx, device=self.device)].view(-1, 3)
 
+    def slot_losses(self) -> np.ndarray:
+        """Per-instance (loss, initial loss) in double, as the stopping tests compare them."""
+        raw = self.state.view(self.M, N.PK_LBFGS_SLOT_BYTES).cpu().numpy()
+        return raw[:, 0:16].copy().view(np.float64)
+
     def slot_records(self) -> np.ndarray:
         """Per-instance (n_iter, n_eval, hist_len, status, reason) read back from the device."""
         raw = self.state.view(self.M, N.PK_LBFGS_SLOT_BYTES).cpu().numpy()

@@ -40,6 +40,17 @@ class Request:
     overlapping_pairs: int = -1
 
 
+def check_request(req: Request, n_max: int) -> None:
+    """Raise ValueError unless the request's poses are (N, 3) with 1 <= N <= n_max and its
+    box is 3 numbers."""
+    shp = tuple(np.shape(req.xyt))
+    if len(shp) != 2 or shp[1] != 3 or not (1 <= shp[0] <= n_max):
+        raise ValueError(f"request {req.rid}: poses of shape {shp}; a slot takes (N, 3) with "
+                         f"1 <= N <= n_max = {n_max}")
+    if np.size(req.h) != 3:
+        raise ValueError(f"request {req.rid}: box h must hold 3 numbers")
+
+
 def slot_bytes(n_max: int, history: int) -> int:
     """HBM bytes one slot holds: x, g (n), s and y history (history * n), record."""
     n = 3 * n_max + 3
@@ -85,7 +96,13 @@ class RelaxServer:
         self.per_slot = per
 
     # ---------------------------------------------------------------- admission
+    def check_request(self, req: Request) -> None:
+        """A slot holds at most n_max trees: a larger request would spill its state and
+        history into the next slot, so it is refused."""
+        check_request(req, self.n_max)
+
     def _admit(self, req: Request, staged: torch.Tensor, now: float) -> None:
+        self.check_request(req)
         s = self.free.pop()
         n = req.xyt.shape[0]
         o = s * self.stride
@@ -174,6 +191,8 @@ class RelaxServer:
         the host admits a request only once its arrival time has passed; admissions made
         while a launch runs are queued on the stream and join the next launch."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            self.check_request(r)
         # inputs resident in HBM before the session starts
         staged = [torch.tensor(np.concatenate([r.xyt.reshape(-1), r.h]).astype(np.float32), device=self.dev)
                   for r in reqs]

@@ -49,3 +49,42 @@ def assert_grad_close(got, ref, rtol, what="grad"):
 
 
 GEOMS = {"tree": reference_tree, "random": lambda: random_polygon(0)}
+
+
+def lut_face_guard(lut, xyt, radius, band_cells=1e-4):
+    """(E, N) mask of trees in a pair whose FP64 relative pose lies within band_cells of a
+    cell face of the table (or of the mirror plane x = 0 of a half table, or of the table's
+    range).  The trilinear gradient jumps across a face, and rounding the relative pose to
+    float32 moves it by at most ~3e-5 cells at these sizes (|dx| <= 2R, coordinates <= 8,
+    a few ulps of each; 2pi/(nt-1) per theta cell), so only such pairs may see the
+    neighbouring cell's gradient.  band_cells = 1e-4 is 3x that bound."""
+    x = np.asarray(xyt, dtype=np.float64)
+    E, N = x.shape[:2]
+    guard = np.zeros((E, N), dtype=bool)
+    cut = 4.0 * radius * radius
+    nx, ny, nt = lut.N_x, lut.N_y, lut.N_theta
+    for e in range(E):
+        X = x[e]
+        dx = X[None, :, 0] - X[:, None, 0]
+        dy = X[None, :, 1] - X[:, None, 1]
+        c, s = np.cos(X[:, 2])[:, None], np.sin(X[:, 2])[:, None]
+        xl, yl = c * dx + s * dy, -s * dx + c * dy
+        th = np.mod(X[None, :, 2] - X[:, None, 2] + np.pi, 2 * np.pi) - np.pi
+        if lut.mirror_x:
+            near_mirror = np.abs(xl) < band_cells * lut.grid_dx
+            th = np.where(xl < 0, -th, th)
+            xl = np.abs(xl)
+        else:
+            near_mirror = np.zeros_like(xl, dtype=bool)
+        fx = (xl - lut.X_min) / (lut.X_max - lut.X_min) * (nx - 1)
+        fy = (yl - lut.Y_min) / (lut.Y_max - lut.Y_min) * (ny - 1)
+        ft = (th - lut.theta_min) / (lut.theta_max - lut.theta_min) * (nt - 1)
+        near = near_mirror.copy()
+        for f in (fx, fy, ft):
+            fr = f - np.floor(f)
+            near |= np.minimum(fr, 1 - fr) < band_cells
+        near |= np.abs(np.abs(th) - np.pi) < band_cells * lut.grid_dtheta   # the +-pi wrap
+        iu = np.triu(np.ones((N, N), dtype=bool), 1) & (dx * dx + dy * dy <= cut)
+        bad = near & iu
+        guard[e] = bad.any(axis=1) | bad.any(axis=0)
+    return guard

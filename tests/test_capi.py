@@ -30,10 +30,24 @@ def test_version():
 
 
 def test_lut_create_rejects_oversized_table():
-    """Tables of 2^28 nodes or more are refused before any device work (no GPU needed)."""
+    """Tables of 2^32 nodes or more are refused before any device work (no GPU needed);
+    the reference's default 900^3 grid is below the limit."""
     import ctypes as C
     L = _native.lib()
     h = C.c_void_p()
-    rc = L.pk_lut_create(C.c_void_p(16), 1024, 1024, 256, -1.0, 1.0, -1.0, 1.0, -3.0, 3.0, 0, 0, C.byref(h))
+    rc = L.pk_lut_create(C.c_void_p(16), 2048, 2048, 1024, -1.0, 1.0, -1.0, 1.0, -3.0, 3.0, 0, 0, C.byref(h))
     assert rc != 0
-    assert b"2^28" in L.pk_last_error()
+    assert b"2^32" in L.pk_last_error()
+    assert 900 ** 3 < 2 ** 32
+
+
+def test_lut_create_rejects_asymmetric_mirror_table():
+    """A mirrored (half) table looks x < 0 up as (-x, -theta); with a theta range that is
+    not symmetric about 0 that index could fall below the first node, so such a table is
+    refused before any device work (no GPU needed)."""
+    import ctypes as C
+    L = _native.lib()
+    h = C.c_void_p()
+    rc = L.pk_lut_create(C.c_void_p(16), 8, 8, 8, -0.1, 1.0, -1.0, 1.0, -3.0, 3.1, 0, 1, C.byref(h))
+    assert rc != 0
+    assert b"symmetric" in L.pk_last_error()

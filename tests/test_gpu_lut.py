@@ -1,11 +1,12 @@
 """Pose-space table: GPU builder against exact FP64 clipping at the grid nodes
 (<= 1e-4 absolute), lookup kernel against the oracle's restatement of the reference
-kernel on the same table, and the trilinear closed form."""
+kernel on the same table (cost to 1e-5 relative, gradient to 1e-5 away from cell faces),
+and tables of the reference's default 900^3 size."""
 import numpy as np
 import pytest
 import torch
 
-from gpu_util import GEOMS, assert_cost_close, assert_grad_close, dense_poses, require_gpu
+from gpu_util import GEOMS, assert_cost_close, assert_grad_close, dense_poses, lut_face_guard, require_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -52,8 +53,16 @@ def test_lookup_kernel_vs_oracle(oracle_mod, mirror):
     overlap_multi_ensemble(xt, c, lut, out_grads=g)
     torch.cuda.synchronize()
     c64, g64 = OL.overlap(geom.radius, x.astype(np.float64))
-    assert_cost_close(c.cpu().numpy(), c64, 1e-4, atol=1e-6)
-    assert_grad_close(g.cpu().numpy(), g64, 1e-3)
+    # float32 relative poses and interpolation: measured <= 1.3e-7 relative on the cost,
+    # <= 1.7e-6 on the gradient (relative to the instance's largest component)
+    assert_cost_close(c.cpu().numpy(), c64, 1e-5, atol=1e-7)
+    guard = lut_face_guard(lut, x, geom.radius)
+    ok = ~guard
+    assert ok.mean() > 0.9
+    gg, gr = g.cpu().numpy(), g64
+    assert_grad_close(np.where(ok[..., None], gg, 0.0), np.where(ok[..., None], gr, 0.0), 1e-5)
+    # near a face the other cell's gradient may be taken: held to the jump between cells
+    assert_grad_close(gg, gr, 5e-2)
 
 
 @pytest.mark.parametrize("mirror", [False, True])
@@ -86,9 +95,56 @@ def test_lookup_table_edges(oracle_mod, mirror):
     c64, g64 = OL.overlap(geom.radius, x.astype(np.float64))
     assert (c64 > 0).sum() >= 12
     # the interpolated value is continuous across cells: exact nodes included
-    assert_cost_close(c.cpu().numpy(), c64, 1e-4, atol=1e-6)
+    assert_cost_close(c.cpu().numpy(), c64, 1e-5, atol=1e-7)
     # on a node the gradient is one-sided, and float and double may pick neighbouring
     # cells; compare it strictly inside the cells only
-    inside = np.array(inside)
+    inside = np.array(inside) & ~lut_face_guard(lut, x, geom.radius).any(axis=1)
     assert inside.sum() >= 10
-    assert_grad_close(g.cpu().numpy()[inside], g64[inside], 1e-3)
+    assert_grad_close(g.cpu().numpy()[inside], g64[inside], 1e-5)
+
+
+@pytest.mark.parametrize("gname", ["tree", "random"])
+def test_reference_default_table_size(oracle_mod, gname):
+    """CollisionCost(use_lookup_table=True) with the reference's defaults (900^3 grid,
+    core/pack_cost.py:315-317; half table for the mirror-symmetric tree, full x range for
+    the random polygon: 817 x 855 x 900 = 6.3e8 nodes after trimming, above 2^28 and 2^29).
+    The table cost is within the grid's interpolation error of the exact area, and lookups
+    equal the oracle's on 2x2x2 sub-blocks read back from the device table."""
+    from packing_amd import pack_cost
+    from packing_amd.context import set_geometry
+    from packing_amd.pack_cuda_lut import overlap_multi_ensemble
+    geom = GEOMS[gname]()
+    set_geometry(geom)
+    cc = pack_cost.CollisionCostOverlappingArea(use_lookup_table=True)
+    cc._ensure_lut_initialized()
+    L = cc._lut
+    nodes = L.N_x * L.N_y * L.N_theta
+    assert L.mirror_x == (gname == "tree")
+    if gname == "random":
+        assert nodes > 4e8
+    x = dense_poses(8, 8, 30)
+    xt = torch.tensor(x, device="cuda")
+    c = torch.empty(8, device="cuda")
+    overlap_multi_ensemble(xt, c, L)
+    c64, _ = oracle_mod.overlap(oracle_mod.OracleGeometry(geom), x.astype(np.float64), False, grads=False)
+    assert np.abs(c.cpu().numpy() - c64).max() <= 1e-4 * np.abs(c64).max()
+    vals = L.lut_d
+    rng = np.random.default_rng(0)
+    checked = 0
+    for _ in range(40):
+        ix, iy, it = (int(v) for v in rng.integers(1, [L.N_x - 2, L.N_y - 2, L.N_theta - 2]))
+        p = [float(A[i]) + f * (float(A[i + 1]) - float(A[i]))
+             for A, i, f in ((L.X, ix, 0.37), (L.Y, iy, 0.61), (L.theta, it, 0.23))]
+        if p[0] < 0 and L.mirror_x:
+            continue
+        pr = np.array([[[0.0, 0.0, 0.0], p]], np.float32)
+        c1 = torch.empty(1, device="cuda")
+        overlap_multi_ensemble(torch.tensor(pr, device="cuda"), c1, L)
+        sub = vals[ix:ix + 2, iy:iy + 2, it:it + 2].cpu().numpy()
+        OL = oracle_mod.OracleLut(L.X[ix:ix + 2], L.Y[iy:iy + 2], L.theta[it:it + 2], sub)
+        v, _ = OL.lookup(*(float(a) for a in pr[0, 1]))
+        assert abs(v - float(c1.item())) <= 1e-6 * max(1.0, abs(v)), (ix, iy, it, v, float(c1.item()))
+        checked += 1
+    assert checked >= 20
+    del cc, L, vals
+    torch.cuda.empty_cache()

@@ -129,22 +129,80 @@ def test_vectorised_equals_individual(name):
 
 
 def test_pack_dynamics_paths_agree():
-    """OptimizerBFGS drop-in path and fused engine give close results after a few iterations."""
+    """OptimizerBFGS on the reference loop (drop-in lbfgs over the cost objects) and on the
+    fused engine (chosen automatically) agree to float32 rounding, fixed and free h."""
     from packing_amd import pack_cost as PC
-    from packing_amd.pack_dynamics import OptimizerBFGS, run_simulation_list
+    from packing_amd.pack_dynamics import OptimizerBFGS
     from packing_amd.solutions import SolutionCollectionSquare
     from gpu_util import dense_poses
     x = torch.tensor(dense_poses(2, 6, 12), device="cuda")
     h = torch.tensor([[3.0, 0.9, 0.9]] * 6, device="cuda")
-    cost = PC.CostCompound(costs=[PC.BoundaryDistanceCost(), PC.CollisionCostOverlappingArea()])
-    sol = SolutionCollectionSquare(x.clone(), h.clone(), use_fixed_h=True)
-    a = OptimizerBFGS(cost=cost, n_iterations=3, history_size=4, max_step=0.1).run_simulation(sol)
-    b = OptimizerBFGS(cost=cost, n_iterations=3, history_size=4, max_step=0.1, fused=True).run_simulation(sol)
-    assert torch.allclose(a.xyt, b.xyt, atol=2e-5)
-    assert torch.equal(a.h, h) and torch.equal(b.h, h)
-    # run_simulation_list groups by tree count and writes back in place
-    sols = [SolutionCollectionSquare(x[:2].clone(), h[:2].clone(), use_fixed_h=True),
-            SolutionCollectionSquare(x[2:3, :7].contiguous().clone(), h[2:3].clone(), use_fixed_h=True)]
-    before = [s.xyt.clone() for s in sols]
-    run_simulation_list(OptimizerBFGS(cost=cost, n_iterations=5, max_step=0.1), sols)
-    assert not torch.equal(before[0], sols[0].xyt)
+    for fixed in (True, False):
+        opt = OptimizerBFGS(n_iterations=3, history_size=4, max_step=0.1)   # default cost
+        sol = SolutionCollectionSquare(x.clone(), h.clone(), use_fixed_h=fixed)
+        assert opt.engine_options(sol) is not None
+        a = opt.run_simulation(sol, engine="reference")
+        b = opt.run_simulation(sol)
+        assert torch.allclose(a.xyt, b.xyt, atol=5e-6), (a.xyt - b.xyt).abs().max()
+        assert torch.allclose(a.h, b.h, atol=5e-6)
+        if fixed:
+            assert torch.equal(a.h, h) and torch.equal(b.h, h)
+        else:
+            assert not torch.equal(b.h, h)
+    # settings the engine does not reproduce take the reference loop
+    assert OptimizerBFGS(tolerance_rel_change=1e-3).engine_options(sol) is None
+    assert OptimizerBFGS(stop_on_cost_increase=True).engine_options(sol) is None
+    assert OptimizerBFGS().engine_options(SolutionCollectionSquare(x.double(), h.double())) is None
+
+
+def test_pack_dynamics_table_cost_vs_oracle(oracle_mod):
+    """The GA's table objective through the unchanged reference API: OptimizerBFGS with
+    Area + Boundary + CollisionCostOverlappingArea(use_lookup_table=True) runs on the fused
+    engine and its first iterates follow the FP64 oracle over the same table."""
+    from packing_amd import pack_cost as PC
+    from packing_amd.context import set_geometry
+    from packing_amd.pack_cuda_lut import LookupTable
+    from packing_amd.pack_dynamics import OptimizerBFGS
+    from packing_amd.solutions import SolutionCollectionSquare
+    from gpu_util import GEOMS, dense_poses
+    geom = GEOMS["tree"]()
+    set_geometry(geom)
+    coll = PC.CollisionCostOverlappingArea(use_lookup_table=True)
+    coll._lut = LookupTable.build_exact(96, 96, 64, full_x=False)
+    cost = PC.CostCompound(costs=[PC.AreaCost(scaling=1e-2), PC.BoundaryDistanceCost(), coll])
+    N = 20
+    x = dense_poses(5, 4, N)
+    h = np.array([[1.2 * np.sqrt(N), 0.6 * np.sqrt(N), 0.6 * np.sqrt(N)]] * 4)
+    opt = OptimizerBFGS(cost=cost, n_iterations=4, history_size=3, max_step=0.1)
+    sol = SolutionCollectionSquare(torch.tensor(x, device="cuda"), torch.tensor(h, device="cuda", dtype=torch.float32),
+                                   use_fixed_h=False)
+    o = opt.engine_options(sol)
+    assert o is not None and o.mode == "lut"
+    r = opt.run_simulation(sol)
+    OL = oracle_mod.OracleLut(coll._lut.X, coll._lut.Y, coll._lut.theta, coll._lut.lut_d.cpu().numpy(), mirror=True)
+    rx, rh, _ = oracle_mod.relax(oracle_mod.OracleGeometry(geom), x.astype(np.float64), h, max_iter=4, m=3,
+                                 max_step=0.1, lut=OL, w_ar=1e-2, fixed_h=False, full=True)
+    assert np.abs(r.xyt.cpu().numpy() - rx).max() < 3e-6
+    assert np.abs(r.h.cpu().numpy() - rh).max() < 3e-6
+
+
+def test_run_simulation_list_one_ragged_batch():
+    """Solutions of different tree counts are relaxed in one ragged batch: each result is
+    bitwise what relaxing that solution alone gives, and is written back in place."""
+    from packing_amd.pack_dynamics import OptimizerBFGS, run_simulation_list
+    from packing_amd.solutions import SolutionCollectionSquare
+    from gpu_util import dense_poses
+    opt = OptimizerBFGS(n_iterations=20, history_size=4, max_step=0.1)
+    sols = []
+    for k, n in enumerate((7, 12, 12, 30)):
+        x = torch.tensor(dense_poses(30 + k, 2, n), device="cuda")
+        h = torch.tensor([[1.2 * n ** 0.5, 0.6 * n ** 0.5, 0.6 * n ** 0.5]] * 2, device="cuda", dtype=torch.float32)
+        sols.append(SolutionCollectionSquare(x, h, use_fixed_h=k % 2 == 0))
+    alone = [opt.run_simulation(s) for s in sols]
+    calls = []
+    orig = opt._run_engine
+    opt._run_engine = lambda ss, o: calls.append(len(ss)) or orig(ss, o)
+    run_simulation_list(opt, sols)
+    assert sorted(calls) == [2, 2]          # one batch for fixed h, one for free h
+    for s, a in zip(sols, alone):
+        assert torch.equal(s.xyt, a.xyt) and torch.equal(s.h, a.h)

@@ -18,3 +18,18 @@ def test_poisson_requests():
     assert n.min() >= 10 and n.max() <= 200
     # log-uniform: the median is near sqrt(10 * 200)
     assert 35 < np.median(n) < 55
+
+
+def test_oversized_request_is_refused():
+    """A request larger than a slot (N > n_max) would overwrite the next slot's state; it
+    is refused before anything is admitted."""
+    import pytest
+    from packing_amd.serving import Request, check_request
+    ok = Request(0, np.zeros((40, 3), np.float32), np.ones(3, np.float32))
+    check_request(ok, 40)
+    with pytest.raises(ValueError):
+        check_request(Request(1, np.zeros((41, 3), np.float32), np.ones(3, np.float32)), 40)
+    with pytest.raises(ValueError):
+        check_request(Request(2, np.zeros((0, 3), np.float32), np.ones(3, np.float32)), 40)
+    with pytest.raises(ValueError):
+        check_request(Request(3, np.zeros((10, 2), np.float32), np.ones(3, np.float32)), 40)
