@@ -179,6 +179,23 @@ int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max, int history, i
                        int block);
 
 /*
+ * Allocation-free relaxation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same parameters and results as pk_relax_step over
+ * fresh slots run to the end (every listed slot starts fresh and runs up to max_iter
+ * iterations; `iters` is ignored), except that the gradient, L-BFGS history and optimizer
+ * records live in the library's static device scratch of PK_RELAX_SCRATCH_BYTES per GPU:
+ * p->g, p->s_hist, p->y_hist and p->state must be NULL.  n_state = total state floats
+ * (the largest xoff[s] + 3 ntree[s] + 3).  Batches needing more than the scratch
+ * (n_slots * PK_LBFGS_SLOT_BYTES + (1 + 2 history) * 4 n_state bytes, each part rounded
+ * up to 256 B) are refused with PK_ERANGE.  The scratch keeps the last call's contents and
+ * is shared by every call on that GPU: calls that overlap on one GPU (other streams,
+ * other threads) race.  The call allocates nothing.
+ */
+#define PK_RELAX_SCRATCH_BYTES (128u << 20)
+int pk_relax_run_noalloc(pk_geometry_t g, const pk_relax_params* p, int n_state, void* stream);
+
+/*
  * Batched L-BFGS for a caller-evaluated objective: the drop-in for
  * lbfgs_torch_parallel.lbfgs without line search (core/lbfgs_torch_parallel.py:411-729),
  * batch-coupled exactly as the reference (the run stops when no system is active; until

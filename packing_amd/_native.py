@@ -17,6 +17,7 @@ PK_MODE_AREA, PK_MODE_SEP, PK_MODE_LUT = 0, 1, 2
 PK_MAX_HISTORY = 16
 PK_LBFGS_SLOT_BYTES = 184
 PK_LBFGS_SYS_BYTES = 48
+PK_RELAX_SCRATCH_BYTES = 128 << 20
 
 _lib = None
 _lock = threading.Lock()
@@ -41,6 +42,7 @@ _SIGS = {
     "pk_lut_build": (i32, [vp, vp, i32, vp, i32, vp, i32, vp, vp]),
     "pk_lut_overlap": (i32, [vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp]),
     "pk_relax_step": (i32, [vp, vp, vp]),
+    "pk_relax_run_noalloc": (i32, [vp, vp, i32, vp]),
     "pk_relax_slot_bytes": (i32, []),
     "pk_relax_lds_bytes": (i32, [vp, i32, i32, i32, i32, i32]),
     "pk_feasible": (i32, [vp, i32, vp, vp, i32, i32, vp, vp]),

@@ -15,7 +15,9 @@
 // when it fits) in LDS and the L-BFGS arithmetic on wave 0; k_relax_w, one wave per
 // instance with the state and history in registers.  Every reduction has a fixed order,
 // so both give the same bits for any workgroup size and may continue each other's runs.
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "pk_eval.h"
@@ -1181,6 +1183,86 @@ static int launch_relax_w(const Geometry* g, const RelaxArgs& A, int rows, hipSt
 
 LutDev make_lut_dev(const void* lut_handle);
 
+// Kernel choice and launch shared by pk_relax_step and pk_relax_run_noalloc.
+static int relax_dispatch(const Geometry* g, const RelaxArgs& A, int block, hipStream_t st) {
+    const pk_relax_params* p = &A.p;
+    const int rows = wave_rows(p->n_tree_max);
+    int blk = block;
+    const bool wave_ok = rows <= W_RMAX && p->history <= W_KH;
+    if (blk == 0) blk = wave_ok ? 1 : 256;
+    if (blk == 1) {
+        if (!wave_ok) {
+            set_error("pk_relax_step: block 1 (one wave per instance) supports up to " +
+                      std::to_string((W_RMAX * WAVE - 3) / 3) + " trees and history " +
+                      std::to_string(W_KH));
+            return PK_ERANGE;
+        }
+        if (p->mode == PK_MODE_AREA) return launch_relax_w<MODE_AREA>(g, A, rows, st);
+        if (p->mode == PK_MODE_SEP) return launch_relax_w<MODE_SEP>(g, A, rows, st);
+        return launch_relax_w<MODE_LUT>(g, A, rows, st);
+    }
+#define PK_RL(M)                                                          \
+    switch (blk) {                                                        \
+        case 64: return launch_relax<M, 64>(g, A, st);                    \
+        case 128: return launch_relax<M, 128>(g, A, st);                  \
+        case 256: return launch_relax<M, 256>(g, A, st);                  \
+        default: set_error("pk_relax_step: block must be 0, 1, 64, 128 or 256"); \
+            return PK_EINVAL;                                             \
+    }
+    if (p->mode == PK_MODE_AREA) { PK_RL(MODE_AREA) }
+    if (p->mode == PK_MODE_SEP) { PK_RL(MODE_SEP) }
+    PK_RL(MODE_LUT)
+#undef PK_RL
+}
+
+// ---------------------------------------------------------------------------------
+// Allocation-free relaxation: the optimizer records, gradients and L-BFGS history of one
+// batch live in a static device array (one copy per GPU, held until the process ends).
+// Host code never passes the array's name to a kernel: its device address is taken once
+// per device with hipGetSymbolAddress and kept in scratch_dev.
+// ---------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PK_RELAX_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static void* scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+static int scratch_address(void** out) {
+    int dev = 0;
+    PK_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+        set_error("pk_relax_run_noalloc: device index out of range");
+        return PK_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_dev[dev] == nullptr) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr) {
+            set_error(std::string("pk_relax_run_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return PK_EHIP;
+        }
+        scratch_dev[dev] = ptr;
+    }
+    *out = scratch_dev[dev];
+    return PK_OK;
+}
+
+// Fresh optimizer records for the launched slots (order[b], b < n_launch; order NULL: b).
+__global__ __launch_bounds__(256) void k_fresh_slots(LbSlot* __restrict__ rec, const int* __restrict__ order,
+                                                     int n_launch) {
+    constexpr int W = sizeof(LbSlot) / sizeof(uint32_t);
+    const long long total = (long long)n_launch * W;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(q / W), w = (int)(q % W);
+        const int s = order ? order[b] : b;
+        reinterpret_cast<uint32_t*>(rec + s)[w] = 0u;
+    }
+}
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
 }  // namespace pk
 
 using namespace pk;
@@ -1216,34 +1298,56 @@ int pk_relax_step(pk_geometry_t h, const pk_relax_params* p, void* stream) {
     A.geom = g->df;
     std::memset(&A.lut, 0, sizeof(A.lut));
     if (p->mode == PK_MODE_LUT) A.lut = make_lut_dev(p->lut);
+    return relax_dispatch(g, A, p->block, reinterpret_cast<hipStream_t>(stream));
+}
+
+int pk_relax_run_noalloc(pk_geometry_t h, const pk_relax_params* p, int n_state, void* stream) {
+    const Geometry* g = reinterpret_cast<const Geometry*>(h);
+    PK_REQUIRE(g != nullptr && p != nullptr, "pk_relax_run_noalloc: NULL argument");
+    PK_REQUIRE(p->mode == PK_MODE_AREA || p->mode == PK_MODE_SEP || p->mode == PK_MODE_LUT,
+               "pk_relax_run_noalloc: bad mode");
+    PK_REQUIRE(p->history >= 1 && p->history <= PK_MAX_HISTORY, "pk_relax_run_noalloc: 1 <= history <= 16");
+    PK_REQUIRE(p->n_tree_max >= 1 && p->n_tree_max <= 4096, "pk_relax_run_noalloc: bad n_tree_max");
+    PK_REQUIRE(p->max_iter >= 0, "pk_relax_run_noalloc: negative iteration count");
+    PK_REQUIRE(p->mode != PK_MODE_LUT || p->lut != nullptr, "pk_relax_run_noalloc: table mode needs a table");
+    PK_REQUIRE(p->hist_lds >= -1 && p->hist_lds <= 1, "pk_relax_run_noalloc: hist_lds must be -1, 0 or 1");
+    PK_REQUIRE(p->g == nullptr && p->s_hist == nullptr && p->y_hist == nullptr && p->state == nullptr,
+               "pk_relax_run_noalloc: g, s_hist, y_hist and state live in the library's scratch; pass NULL");
+    PK_REQUIRE(p->n_slots >= 0 && n_state >= 0 && p->n_launch <= p->n_slots,
+               "pk_relax_run_noalloc: bad n_slots, n_launch or n_state");
+    const size_t rec_b = round256((size_t)p->n_slots * sizeof(LbSlot));
+    const size_t vec_b = round256((size_t)n_state * sizeof(float));
+    const size_t his_b = round256((size_t)p->history * (size_t)n_state * sizeof(float));
+    const size_t need = rec_b + vec_b + 2 * his_b;
+    if (need > (size_t)PK_RELAX_SCRATCH_BYTES) {
+        set_error("pk_relax_run_noalloc: the batch needs " + std::to_string(need) +
+                  " B of scratch (records, gradient, history), the library holds " +
+                  std::to_string((size_t)PK_RELAX_SCRATCH_BYTES) + " B; split it or use pk_relax_step");
+        return PK_ERANGE;
+    }
+    if (p->n_launch <= 0 || p->max_iter == 0) return PK_OK;
+    PK_REQUIRE(p->xoff && p->ntree && p->x, "pk_relax_run_noalloc: missing x, xoff or ntree");
+    void* base = nullptr;
+    const int rc = scratch_address(&base);
+    if (rc != PK_OK) return rc;
+    char* b = static_cast<char*>(base);
+    RelaxArgs A;
+    A.p = *p;
+    A.p.state = b;
+    A.p.g = reinterpret_cast<float*>(b + rec_b);
+    A.p.s_hist = reinterpret_cast<float*>(b + rec_b + vec_b);
+    A.p.y_hist = reinterpret_cast<float*>(b + rec_b + vec_b + his_b);
+    A.p.iters = p->max_iter;   // fresh slots, run to the end
+    if (A.p.max_eval <= 0) A.p.max_eval = p->max_iter * 5 / 4;
+    A.geom = g->df;
+    std::memset(&A.lut, 0, sizeof(A.lut));
+    if (p->mode == PK_MODE_LUT) A.lut = make_lut_dev(p->lut);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int rows = wave_rows(p->n_tree_max);
-    int blk = p->block;
-    const bool wave_ok = rows <= W_RMAX && p->history <= W_KH;
-    if (blk == 0) blk = wave_ok ? 1 : 256;
-    if (blk == 1) {
-        if (!wave_ok) {
-            set_error("pk_relax_step: block 1 (one wave per instance) supports up to " +
-                      std::to_string((W_RMAX * WAVE - 3) / 3) + " trees and history " +
-                      std::to_string(W_KH));
-            return PK_ERANGE;
-        }
-        if (p->mode == PK_MODE_AREA) return launch_relax_w<MODE_AREA>(g, A, rows, st);
-        if (p->mode == PK_MODE_SEP) return launch_relax_w<MODE_SEP>(g, A, rows, st);
-        return launch_relax_w<MODE_LUT>(g, A, rows, st);
-    }
-#define PK_RL(M)                                                          \
-    switch (blk) {                                                        \
-        case 64: return launch_relax<M, 64>(g, A, st);                    \
-        case 128: return launch_relax<M, 128>(g, A, st);                  \
-        case 256: return launch_relax<M, 256>(g, A, st);                  \
-        default: set_error("pk_relax_step: block must be 0, 1, 64, 128 or 256"); \
-            return PK_EINVAL;                                             \
-    }
-    if (p->mode == PK_MODE_AREA) { PK_RL(MODE_AREA) }
-    if (p->mode == PK_MODE_SEP) { PK_RL(MODE_SEP) }
-    PK_RL(MODE_LUT)
-#undef PK_RL
+    const int blocks = std::max(1, std::min((p->n_launch * (int)(sizeof(LbSlot) / 4) + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_fresh_slots, dim3(blocks), dim3(256), 0, st, reinterpret_cast<LbSlot*>(b), p->order,
+                       p->n_launch);
+    PK_CHECK_HIP(hipGetLastError());
+    return relax_dispatch(g, A, p->block, st);
 }
 
 }  // extern "C"

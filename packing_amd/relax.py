@@ -221,6 +221,20 @@ class RelaxBatch:
         for _ in range(n_launch):
             self.launch(k, stream=stream)
 
+    def run_noalloc(self, stream=None) -> None:
+        """Relax the batch from its initial poses to the end in one call that allocates
+        nothing: gradient, history and optimizer records in the library's static scratch
+        (pk_relax_run_noalloc); poses, h, loss and status land in this batch's tensors.
+        The batch's own g, history and records are not touched."""
+        p = self._params(self.opts.max_iter)
+        p.g = p.s_hist = p.y_hist = p.state = None
+        p.stats = None
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(st):
+            torch.mul(self.x0, 1.0, out=self.x)   # a kernel: nothing is allocated
+        N.check(N.lib().pk_relax_run_noalloc(self._dg.handle, C.byref(p), self.n_state,
+                                             N.stream_handle(stream)), "pk_relax_run_noalloc")
+
     def run_plain(self, instances=None, stream=None) -> None:
         """Baseline path: one launch per instance per iteration."""
         idx = range(self.M) if instances is None else instances
