@@ -221,12 +221,53 @@ typedef struct {
     int* counters;
     double lr, tol_grad, tol_change, tol_rel, max_step;   /* max_step <= 0: no clamp     */
     int max_eval, stop_on_increase;
+    int line_search;      /* 0: no line search (pk_lbfgs_step moves x); 1: strong Wolfe
+                             (pk_lbfgs_ls_*), evaluation counts per system               */
 } pk_lbfgs_batch;
 
 int pk_lbfgs_sys_bytes(void);
 int pk_lbfgs_begin(const pk_lbfgs_batch* b, void* stream);
 int pk_lbfgs_direction(const pk_lbfgs_batch* b, int it, void* stream);
 int pk_lbfgs_step(const pk_lbfgs_batch* b, int it, void* stream);
+
+/*
+ * Strong-Wolfe line search of the batched L-BFGS (lbfgs(..., line_search_fn=
+ * 'strong_wolfe'), core/lbfgs_torch_parallel.py:30-407 and the line-search branch of
+ * :606-622), with the reference's batch-shared search loop: the host drives the loop,
+ * every search decision runs on the device, one wave per system.  After
+ * pk_lbfgs_direction(it) (line_search = 1), with c = the number of flagged rows read
+ * from *count after each call that sets it:
+ *   pk_lbfgs_ls_begin                 systems still active start a search (flagged)
+ *   evaluate the flagged rows of x into f, g; pk_lbfgs_ls_gtd; ls_iter = 0
+ *   while ls_iter < 25: pk_lbfgs_ls_bracket(ls_iter); stop if c == 0;
+ *                       evaluate, pk_lbfgs_ls_gtd, ++ls_iter
+ *   pk_lbfgs_ls_zoom_begin(ls_iter)
+ *   while c > 0 and ls_iter < 25: pk_lbfgs_ls_zoom_trial; stop if c == 0;
+ *                       evaluate, pk_lbfgs_ls_gtd, ++ls_iter, pk_lbfgs_ls_zoom_update
+ *   pk_lbfgs_ls_end                   accepted step: x += t d, loss, gradient, counts
+ * Sizes: f (M) dtype; g, x, g_prev (M, n) dtype; bg (M, 2, n) dtype; sys
+ * M * PK_LBFGS_LS_BYTES; flag (M) int32 (1 = row evaluated at the next objective call);
+ * count [1] int32.  Scalars of the search are double.
+ */
+#define PK_LBFGS_LS_BYTES 160
+typedef struct {
+    void* f;
+    void* g;
+    void* x;
+    void* g_prev;
+    void* bg;
+    void* sys;
+    int32_t* flag;
+    int32_t* count;
+} pk_lbfgs_ls;
+
+int pk_lbfgs_ls_begin(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, void* stream);
+int pk_lbfgs_ls_gtd(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, void* stream);
+int pk_lbfgs_ls_bracket(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, int ls_iter, void* stream);
+int pk_lbfgs_ls_zoom_begin(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, int ls_iter, void* stream);
+int pk_lbfgs_ls_zoom_trial(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, void* stream);
+int pk_lbfgs_ls_zoom_update(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, void* stream);
+int pk_lbfgs_ls_end(const pk_lbfgs_batch* b, const pk_lbfgs_ls* ls, void* stream);
 
 /*
  * FP64 zero-overlap check of final packings: out_pairs[e] = number of tree pairs of

@@ -314,3 +314,242 @@ def lattice_axes(kind, h, aspect=None, angle=None):
         b = h[:, 0] * np.asarray(aspect)
         return np.stack([h[:, 0], 0 * h[:, 0], b * np.cos(ang), b * np.sin(ang)], -1)
     raise ValueError(kind)
+
+
+# ---------------------------------------------------------------------------------
+# Batched L-BFGS with the strong-Wolfe line search, in numpy (float64): a restatement of
+# core/lbfgs_torch_parallel.py:30-407 (_cubic_interpolate_batch, _strong_wolfe_batched)
+# and of the line-search branch of lbfgs (:411-729).  func(x_rows) -> (loss, grad) must
+# not depend on the row index: the search evaluates only the rows still searching, as a
+# smaller batch, as the reference does.  `calls` (optional list) receives the number of
+# rows of every objective call.
+# ---------------------------------------------------------------------------------
+def _cubic_interpolate(x1, f1, g1, x2, f2, g2, lo, hi):
+    """core/lbfgs_torch_parallel.py:30-66"""
+    with np.errstate(all="ignore"):
+        d1 = g1 + g2 - 3 * (f1 - f2) / (x1 - x2)
+        d2s = d1 ** 2 - g1 * g2
+        valid = d2s >= 0
+        d2 = np.sqrt(np.maximum(d2s, 0))
+        p1 = x2 - (x2 - x1) * ((g2 + d2 - d1) / (g2 - g1 + 2 * d2))
+        p2 = x1 - (x1 - x2) * ((g1 + d2 - d1) / (g1 - g2 + 2 * d2))
+        mp = np.where(x1 <= x2, p1, p2)
+        return np.where(valid, np.minimum(np.maximum(mp, lo), hi), (lo + hi) / 2.0)
+
+
+def _strong_wolfe(func, x, t, d, f, g, gtd, calls, c1=1e-4, c2=0.9, tol=1e-9, max_ls=25):
+    """core/lbfgs_torch_parallel.py:70-407 on the rows given (all searching)."""
+    M = x.shape[0]
+    d_norm = np.abs(d).max(axis=1)
+    g = g.copy()
+    t = t.copy()
+    x_new = x + t[:, None] * d
+    f_new, g_new = func(x_new)
+    calls.append(M)
+    f_new, g_new = np.array(f_new, dtype=np.float64), np.array(g_new, dtype=np.float64)
+    evals = np.ones(M, dtype=np.int64)
+    gtd_new = (g_new * d).sum(1)
+    active = np.ones(M, dtype=bool)
+    t_prev = np.zeros(M)
+    f_prev, g_prev, gtd_prev = f.copy(), g.copy(), gtd.copy()
+    bt, bf = np.zeros((M, 2)), np.zeros((M, 2))
+    bg, bgtd = np.zeros((M, 2, x.shape[1])), np.zeros((M, 2))
+    bsize = np.zeros(M, dtype=np.int64)
+    done = np.zeros(M, dtype=bool)
+    ls_iter = 0
+
+    def set2(m, t0, f0, g0, gt0, t1, f1, g1, gt1):
+        bsize[m] = 2
+        bt[m, 0], bt[m, 1] = t0[m], t1[m]
+        bf[m, 0], bf[m, 1] = f0[m], f1[m]
+        bg[m, 0], bg[m, 1] = g0[m], g1[m]
+        bgtd[m, 0], bgtd[m, 1] = gt0[m], gt1[m]
+
+    while active.any() and ls_iter < max_ls:
+        armijo_fail = f_new > (f + c1 * t * gtd)
+        not_dec = (ls_iter > 1) & (f_new >= f_prev)
+        nb = active & (armijo_fail | not_dec) & (bsize == 0)
+        if nb.any():
+            set2(nb, t_prev, f_prev, g_prev, gtd_prev, t, f_new, g_new, gtd_new)
+            active[nb] = False
+        ws = (np.abs(gtd_new) <= -c2 * gtd) & active
+        if ws.any():
+            bsize[ws] = 1
+            bt[ws, 0], bf[ws, 0], bg[ws, 0] = t[ws], f_new[ws], g_new[ws]
+            done[ws] = True
+            active[ws] = False
+        pc = (gtd_new >= 0) & active
+        if pc.any():
+            set2(pc, t_prev, f_prev, g_prev, gtd_prev, t, f_new, g_new, gtd_new)
+            active[pc] = False
+        if not active.any():
+            break
+        lo, hi = t + 0.01 * (t - t_prev), t * 10
+        a = active
+        t[a] = _cubic_interpolate(t_prev[a], f_prev[a], gtd_prev[a], t[a], f_new[a], gtd_new[a], lo[a], hi[a])
+        t_prev, f_prev, g_prev, gtd_prev = t.copy(), f_new.copy(), g_new.copy(), gtd_new.copy()
+        x_new[a] = x[a] + t[a][:, None] * d[a]
+        fb, gb = func(x_new[a])
+        calls.append(int(a.sum()))
+        f_new[a], g_new[a] = fb, gb
+        evals[a] += 1
+        gtd_new[a] = (g_new[a] * d[a]).sum(1)
+        ls_iter += 1
+    nob = (bsize == 0) & (ls_iter == max_ls)
+    if nob.any():
+        bsize[nob] = 2
+        bt[nob, 0], bt[nob, 1] = 0, t[nob]
+        bf[nob, 0], bf[nob, 1] = f[nob], f_new[nob]
+        bg[nob, 0], bg[nob, 1] = g[nob], g_new[nob]
+    active = (bsize == 2) & ~done
+    insuf = np.zeros(M, dtype=bool)
+    low = (bf[:, 0] <= bf[:, 1]).astype(np.int64)      # as the reference initialises it
+    high = 1 - low
+    rows = np.arange(M)
+    while active.any() and ls_iter < max_ls:
+        width = np.abs(bt[:, 1] - bt[:, 0])
+        active[(width * d_norm < tol) & active] = False
+        if not active.any():
+            break
+        t_new = np.zeros(M)
+        bmin, bmax = bt.min(1), bt.max(1)
+        a = active
+        t_new[a] = _cubic_interpolate(bt[a, 0], bf[a, 0], bgtd[a, 0], bt[a, 1], bf[a, 1], bgtd[a, 1],
+                                      bmin[a], bmax[a])
+        eps = 0.1 * width
+        dist = np.minimum(bmax - t_new, t_new - bmin)
+        close = (dist < eps) & active
+        adj = close & (insuf | (t_new >= bmax) | (t_new <= bmin))
+        if adj.any():
+            to_max = np.abs(t_new - bmax) < np.abs(t_new - bmin)
+            t_new[adj] = np.where(to_max, bmax - eps, bmin + eps)[adj]
+            insuf[adj] = False
+        insuf[close & ~adj] = True
+        insuf[~close] = False
+        x_new[a] = x[a] + t_new[a][:, None] * d[a]
+        fb, gb = func(x_new[a])
+        calls.append(int(a.sum()))
+        f_new[a], g_new[a] = fb, gb
+        evals[a] += 1
+        gtd_new[a] = (g_new[a] * d[a]).sum(1)
+        ls_iter += 1
+        low_f = bf[rows, low]
+        todo1 = active & ((f_new > (f + c1 * t_new * gtd)) | (f_new >= low_f))
+        if todo1.any():
+            hp = high[todo1]
+            idx = rows[todo1]
+            bt[idx, hp], bf[idx, hp] = t_new[todo1], f_new[todo1]
+            bg[idx, hp], bgtd[idx, hp] = g_new[todo1], gtd_new[todo1]
+            m01 = bf[:, 0] <= bf[:, 1]
+            low[todo1] = np.where(m01[todo1], 0, 1)
+            high[todo1] = 1 - low[todo1]
+        todo2 = active & ~todo1
+        ws = todo2 & (np.abs(gtd_new) <= -c2 * gtd)
+        done[ws] = True
+        active[ws] = False
+        upd = todo2 & ~ws
+        if upd.any():
+            idx = rows[upd]
+            swap = gtd_new[upd] * (bt[idx, high[upd]] - bt[idx, low[upd]]) >= 0
+            for m in idx[swap]:
+                lp, hp = low[m], high[m]
+                bt[m, hp], bf[m, hp], bg[m, hp], bgtd[m, hp] = bt[m, lp], bf[m, lp], bg[m, lp].copy(), bgtd[m, lp]
+        if todo2.any():
+            idx = rows[todo2]
+            lp = low[todo2]
+            bt[idx, lp], bf[idx, lp] = t_new[todo2], f_new[todo2]
+            bg[idx, lp], bgtd[idx, lp] = g_new[todo2], gtd_new[todo2]
+    s1 = bsize == 1
+    s2 = ~s1
+    t[s1], f_new[s1], g_new[s1] = bt[s1, 0], bf[s1, 0], bg[s1, 0]
+    li = low[s2]
+    t[s2], f_new[s2], g_new[s2] = bt[rows[s2], li], bf[rows[s2], li], bg[rows[s2], li]
+    return f_new, g_new, t, evals
+
+
+def lbfgs_wolfe(func, x0, lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-7, tolerance_change=1e-9,
+                tolerance_rel_change=0.0, stop_on_cost_increase=False, history_size=100, max_step=0.5,
+                calls=None):
+    """lbfgs(..., line_search_fn='strong_wolfe') of core/lbfgs_torch_parallel.py:411-729."""
+    calls = [] if calls is None else calls
+    if max_eval is None:
+        max_eval = max_iter * 5 // 4
+    x = np.array(x0, dtype=np.float64)
+    M, n = x.shape
+    m = history_size
+    loss, g = func(x)
+    calls.append(M)
+    loss, g = np.array(loss, dtype=np.float64), np.array(g, dtype=np.float64)
+    fe = np.ones(M, dtype=np.int64)
+    active = ~(np.abs(g).max(1) <= tolerance_grad)
+    if not active.any():
+        return x
+    S, Y = np.zeros((M, m, n)), np.zeros((M, m, n))
+    ro, hl, H = np.zeros((M, m)), np.zeros(M, dtype=np.int64), np.ones(M)
+    prev_loss0 = loss.copy()
+    it = 0
+    while it < max_iter and active.any():
+        it += 1
+        if it == 1:
+            d = -g.copy()
+            H[:] = 1.0
+        else:
+            d = np.zeros_like(g)
+            for a in range(M):
+                q = -g[a].copy()
+                al = np.zeros(m)
+                for i in range(m - 1, -1, -1):
+                    if i < hl[a]:
+                        al[i] = (S[a, i] * q).sum() * ro[a, i]
+                        q = q - Y[a, i] * al[i]
+                r = q * H[a]
+                for i in range(m):
+                    if i < hl[a]:
+                        be = (Y[a, i] * r).sum() * ro[a, i]
+                        r = r + S[a, i] * (al[i] - be)
+                d[a] = r
+        g_prev, l_prev = g.copy(), loss.copy()
+        if it == 1:
+            with np.errstate(divide="ignore"):
+                t = np.minimum(1.0, 1.0 / np.abs(g).sum(1)) * lr
+            t[~active] = 0
+        else:
+            t = np.full(M, float(lr))
+        if max_step is not None:
+            pc = np.abs(d * t[:, None]).max(1)
+            sc = pc > max_step
+            t[sc] *= max_step / pc[sc]
+        gtd = (g * d).sum(1)
+        active &= ~(gtd > -tolerance_change)
+        if not active.any():
+            break
+        a = active
+        fl, gl, tl, ev = _strong_wolfe(func, x[a], t[a], d[a], loss[a], g[a], gtd[a], calls)
+        loss[a], g[a], t[a] = fl, gl, tl
+        lse = np.zeros(M, dtype=np.int64)
+        lse[a] = ev
+        x[a] = x[a] + t[a][:, None] * d[a]
+        fe += lse
+        if it > 1:
+            y, s = g - g_prev, d * t[:, None]
+            ys = (y * s).sum(1)
+            upd = active & (ys > 1e-10)
+            for k in np.where(upd)[0]:
+                H[k] = ys[k] / (y[k] * y[k]).sum()
+                if hl[k] < m:
+                    S[k, hl[k]], Y[k, hl[k]], ro[k, hl[k]] = s[k], y[k], 1.0 / ys[k]
+                    hl[k] += 1
+                else:
+                    S[k, :-1], Y[k, :-1], ro[k, :-1] = S[k, 1:].copy(), Y[k, 1:].copy(), ro[k, 1:].copy()
+                    S[k, -1], Y[k, -1], ro[k, -1] = s[k], y[k], 1.0 / ys[k]
+        active &= ~(np.abs(g).max(1) <= tolerance_grad)
+        active &= ~(fe >= max_eval)
+        pch = np.abs(d * t[:, None]).max(1)
+        active &= ~((pch <= tolerance_change) & active)
+        lc = np.abs(loss - l_prev)
+        active &= ~((lc < tolerance_change) & active)
+        if tolerance_rel_change > 0 and it > history_size:
+            active &= ~((lc / (np.abs(l_prev) + 1e-12) < tolerance_rel_change) & active)
+        if stop_on_cost_increase:
+            active &= ~((loss > prev_loss0) & active)
+    return x

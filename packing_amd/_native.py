@@ -17,6 +17,7 @@ PK_MODE_AREA, PK_MODE_SEP, PK_MODE_LUT = 0, 1, 2
 PK_MAX_HISTORY = 16
 PK_LBFGS_SLOT_BYTES = 184
 PK_LBFGS_SYS_BYTES = 48
+PK_LBFGS_LS_BYTES = 160
 PK_RELAX_SCRATCH_BYTES = 128 << 20
 
 _lib = None
@@ -50,6 +51,13 @@ _SIGS = {
     "pk_lbfgs_begin": (i32, [vp, vp]),
     "pk_lbfgs_direction": (i32, [vp, i32, vp]),
     "pk_lbfgs_step": (i32, [vp, i32, vp]),
+    "pk_lbfgs_ls_begin": (i32, [vp, vp, vp]),
+    "pk_lbfgs_ls_gtd": (i32, [vp, vp, vp]),
+    "pk_lbfgs_ls_bracket": (i32, [vp, vp, i32, vp]),
+    "pk_lbfgs_ls_zoom_begin": (i32, [vp, vp, i32, vp]),
+    "pk_lbfgs_ls_zoom_trial": (i32, [vp, vp, vp]),
+    "pk_lbfgs_ls_zoom_update": (i32, [vp, vp, vp]),
+    "pk_lbfgs_ls_end": (i32, [vp, vp, vp]),
     "pk_lap_hungarian": (i32, [i32, vp, i32, i32, vp, vp, vp, vp]),
     "pk_diversity": (i32, [vp, vp, i32, i32, i32, C.c_float, C.c_float, i32, vp, vp]),
 }
@@ -75,7 +83,13 @@ class LbfgsBatch(C.Structure):
                 ("loss", vp), ("d", vp), ("g_prev", vp), ("s_hist", vp), ("y_hist", vp), ("ro", vp),
                 ("al", vp), ("sys", vp), ("counters", vp), ("lr", C.c_double), ("tol_grad", C.c_double),
                 ("tol_change", C.c_double), ("tol_rel", C.c_double), ("max_step", C.c_double),
-                ("max_eval", i32), ("stop_on_increase", i32)]
+                ("max_eval", i32), ("stop_on_increase", i32), ("line_search", i32)]
+
+
+class LbfgsLs(C.Structure):
+    """ctypes mirror of pk_lbfgs_ls (include/packing.h)."""
+    _fields_ = [("f", vp), ("g", vp), ("x", vp), ("g_prev", vp), ("bg", vp), ("sys", vp), ("flag", vp),
+                ("count", vp)]
 
 
 class PackingError(RuntimeError):
@@ -88,6 +102,11 @@ def lib():
         return _lib
     with _lock:
         if _lib is None:
+            # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) as
+            # the one this library links; loading torch first makes both use that one
+            # runtime (loading ours first leaves torch and this library on different
+            # runtime instances, and device allocations then fail)
+            import torch  # noqa: F401
             if not os.path.exists(LIB_PATH):
                 raise PackingError(
                     f"{LIB_PATH} is missing: build it with `python -m packing_amd._build` "

@@ -73,3 +73,75 @@ def test_float32_converges_to_minimiser():
     x = torch.zeros(M, n, device="cuda")
     lbfgs(f, x, max_iter=60, history_size=8, max_step=None, tolerance_grad=0, tolerance_change=0)
     np.testing.assert_allclose(x.cpu().numpy(), (b / d).cpu().numpy(), atol=1e-4)
+
+
+W = np.load(os.path.join(HERE, "golden", "lbfgs_wolfe_golden.npz"))
+
+
+def torch_wolfe_fns(calls):
+    A = torch.tensor(W["quad_A"], device="cuda")
+    b = torch.tensor(W["quad_b"], device="cuda")
+
+    def quad_shared(x, _m=True):
+        calls.append(x.shape[0])
+        Ax = x @ A.T
+        return 0.5 * (x * Ax).sum(1) - (b * x).sum(1), Ax - b
+
+    fns = torch_fns()
+
+    def counted(f):
+        def g(x, _m=True):
+            calls.append(x.shape[0])
+            return f(x, _m)
+        return g
+
+    return {"quad_shared": quad_shared, "rosen": counted(fns["rosen"]), "abs": counted(fns["abs"])}
+
+
+@pytest.mark.parametrize("k", range(len(W["meta"])))
+def test_lbfgs_strong_wolfe_matches_reference(oracle_mod, k):
+    """lbfgs(..., line_search_fn='strong_wolfe') on the device against iterates recorded
+    from the reference (float64): same iterates (bound: 10x the oracle's own response to
+    x0 changes of a few ulps) and, where that response leaves the branch decisions
+    unchanged, the same sequence of objective calls (rows per call)."""
+    from packing_amd.lbfgs_torch_parallel import lbfgs
+    from test_oracle_lbfgs import WFNS
+    name, opts = str(W["meta"][k]).split("|", 1)
+    opts = ast.literal_eval(opts)
+    x0 = W[f"x0_{k}"]
+    calls = []
+    x = torch.tensor(x0, device="cuda")
+    lbfgs(torch_wolfe_fns(calls)[name], x, **opts)
+    ref = W[f"x_{k}"]
+    o = dict(opts)
+    o.pop("line_search_fn")
+    c0 = []
+    xo = oracle_mod.lbfgs_wolfe(WFNS[name], x0, calls=c0, **o)
+    spread, stable = 0.0, True
+    for eps in (1e-15, -1e-15, 3e-15):
+        cp = []
+        xp = oracle_mod.lbfgs_wolfe(WFNS[name], x0 * (1 + eps), calls=cp, **o)
+        spread = max(spread, np.abs(xp - xo).max())
+        stable &= cp == c0
+    tol = max(1e-9, 10 * spread)
+    assert np.abs(x.cpu().numpy() - ref).max() <= tol, (np.abs(x.cpu().numpy() - ref).max(), tol)
+    if stable:
+        assert calls == list(W[f"calls_{k}"]), (calls, list(W[f"calls_{k}"]))
+
+
+def test_strong_wolfe_float32_converges():
+    """float32 with the line search: a well-conditioned batched quadratic reaches A^-1 b in
+    fewer iterations than without it."""
+    from packing_amd.lbfgs_torch_parallel import lbfgs
+    rng = np.random.default_rng(4)
+    M, n = 64, 30
+    dd = torch.tensor(rng.uniform(1.0, 4.0, n), device="cuda", dtype=torch.float32)
+    b = torch.tensor(rng.normal(size=n), device="cuda", dtype=torch.float32)
+
+    def f(x, _m=True):
+        return 0.5 * (dd * x * x).sum(1) - (b * x).sum(1), dd * x - b
+
+    x = torch.zeros(M, n, device="cuda")
+    lbfgs(f, x, max_iter=25, history_size=8, max_step=None, tolerance_grad=0, tolerance_change=0,
+          line_search_fn="strong_wolfe")
+    np.testing.assert_allclose(x.cpu().numpy(), np.broadcast_to((b / dd).cpu().numpy(), (M, n)), atol=1e-4)

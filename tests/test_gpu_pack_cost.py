@@ -206,3 +206,40 @@ def test_run_simulation_list_one_ragged_batch():
     assert sorted(calls) == [2, 2]          # one batch for fixed h, one for free h
     for s, a in zip(sols, alone):
         assert torch.equal(s.xyt, a.xyt) and torch.equal(s.h, a.h)
+
+
+def test_pack_dynamics_line_search_vs_oracle(oracle_mod):
+    """OptimizerBFGS(use_line_search=True) (the reference loop with the device strong-Wolfe
+    search) on FP64 packings follows the oracle's strong-Wolfe L-BFGS over the FP64
+    overlap + boundary cost for its first iterations."""
+    from packing_amd import pack_cost as PC
+    from packing_amd.context import set_geometry
+    from packing_amd.pack_dynamics import OptimizerBFGS
+    from packing_amd.solutions import SolutionCollectionSquare
+    from gpu_util import GEOMS, dense_poses
+    geom = GEOMS["tree"]()
+    set_geometry(geom)
+    og = oracle_mod.OracleGeometry(geom)
+    N, E = 12, 4
+    x = dense_poses(9, E, N).astype(np.float64)
+    h = np.array([[1.2 * np.sqrt(N), 0.6 * np.sqrt(N), 0.6 * np.sqrt(N)]] * E)
+    cost = PC.CostCompound(costs=[PC.BoundaryDistanceCost(), PC.CollisionCostOverlappingArea()])
+    opt = OptimizerBFGS(cost=cost, n_iterations=4, history_size=3, max_step=0.1, use_line_search=True)
+    sol = SolutionCollectionSquare(torch.tensor(x, device="cuda"), torch.tensor(h, device="cuda"), use_fixed_h=True)
+    assert opt.engine_options(sol) is None
+    r = opt.run_simulation(sol)
+
+    def f(rows):
+        xyt = rows[:, :3 * N].reshape(-1, N, 3)
+        hh = rows[:, 3 * N:]
+        co, go = oracle_mod.overlap(og, xyt, False)
+        cb, gb, _ = oracle_mod.boundary(og, xyt, hh)
+        g = np.zeros_like(rows)
+        g[:, :3 * N] = (go + gb).reshape(len(rows), -1)
+        return co + cb, g
+
+    x0 = np.concatenate([x.reshape(E, -1), h], 1)
+    ref = oracle_mod.lbfgs_wolfe(f, x0, max_iter=4, history_size=3, max_step=0.1, tolerance_grad=0.0,
+                                 tolerance_change=0.0)
+    assert np.abs(r.xyt.cpu().numpy().reshape(E, -1) - ref[:, :3 * N]).max() < 1e-9
+    assert not np.allclose(ref[:, :3 * N], x0[:, :3 * N])

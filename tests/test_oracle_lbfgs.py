@@ -47,3 +47,41 @@ def test_lbfgs_matches_reference(oracle_mod, k):
     xp = oracle_mod.lbfgs(FNS[name], x0 * (1 + 1e-15), **opts)
     tol = max(1e-9, 10 * np.abs(xp - x).max())
     assert np.abs(x - ref).max() <= tol, (np.abs(x - ref).max(), tol)
+
+
+W = np.load(os.path.join(HERE, "golden", "lbfgs_wolfe_golden.npz"))
+
+
+def quad_shared(x):
+    Ax = x @ W["quad_A"].T
+    return 0.5 * (x * Ax).sum(1) - (W["quad_b"] * x).sum(1), Ax - W["quad_b"]
+
+
+WFNS = {"quad_shared": quad_shared, "rosen": rosen, "abs": absf}
+
+
+@pytest.mark.parametrize("k", range(len(W["meta"])))
+def test_lbfgs_wolfe_matches_reference(oracle_mod, k):
+    """The oracle's strong-Wolfe L-BFGS against iterates recorded from the reference, and
+    the same sequence of objective calls (rows per call: the batch-shared line-search
+    loop evaluates only the rows still searching)."""
+    name, opts = str(W["meta"][k]).split("|", 1)
+    opts = ast.literal_eval(opts)
+    opts.pop("line_search_fn")
+    x0 = W[f"x0_{k}"]
+    calls = []
+    x = oracle_mod.lbfgs_wolfe(WFNS[name], x0, calls=calls, **opts)
+    ref = W[f"x_{k}"]
+    # the bound is 10x the run's own response to relative changes of x0 of a few ulps:
+    # on the nonsmooth objective exact ties decide branches, and where they do the
+    # response (and the bound) is large, i.e. that case pins nothing beyond it
+    spread, stable = 0.0, True
+    for eps in (1e-15, -1e-15, 3e-15):
+        cp = []
+        xp = oracle_mod.lbfgs_wolfe(WFNS[name], x0 * (1 + eps), calls=cp, **opts)
+        spread = max(spread, np.abs(xp - x).max())
+        stable &= cp == calls
+    tol = max(1e-9, 10 * spread)
+    assert np.abs(x - ref).max() <= tol, (np.abs(x - ref).max(), tol)
+    if stable:
+        assert calls == list(W[f"calls_{k}"]), (calls, list(W[f"calls_{k}"]))
