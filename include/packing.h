@@ -115,6 +115,20 @@ int pk_lut_build(pk_geometry_t g, const float* X, int nx, const float* Y, int ny
                  int nt, float* out_vals, void* stream);
 
 /*
+ * Fills vals (nx*ny*nt, float32, device, theta fastest) with the signed separation
+ * distance of the polygon at (X[ix], Y[iy], T[it]) from the polygon at the origin
+ * (theta 0), computed in double: the distance of the point to the boundary of the union
+ * over piece pairs of the convex Minkowski differences, positive inside (penetration
+ * depth), negative outside (clearance).  Replaces LookupTable.build_from_function driven
+ * by pack_minkowski.separation_distance (core/pack_cost.py:1170-1200,
+ * core/pack_minkowski.py:33-266), with the table of CollisionCostExactSeparation as its
+ * caller.  Synchronizes the stream; PK_ERANGE if a theta node's zone boundary has more
+ * parts than the builder holds.
+ */
+int pk_sep_build(pk_geometry_t g, const float* X, int nx, const float* Y, int ny, const float* T, int nt,
+                 float* out_vals, void* stream);
+
+/*
  * Table-based cost: sum over tree pairs i<j within (2R) of the trilinear table value of
  * j's pose relative to i, gradient to both trees.  Replaces
  * pack_cuda_lut.overlap_multi_ensemble (core/pack_cuda_lut.py:783-862) and its kernel

@@ -933,3 +933,181 @@ void or_relax(const or_geom *g, const or_lut *lut, double *xyt, double *h, int n
     }
     (void)n_threads;
 }
+
+/* ---------------------------------------------------------------------------
+ * Signed separation distance of two copies of the polygon: core/pack_minkowski.py:
+ * 33-64 (convex Minkowski difference as the hull of vertex differences), 117-145 (the
+ * forbidden zone = union over piece pairs), 210-266 (signed distance of the second
+ * copy's centre to the zone's boundary: + inside = overlap, - outside = clearance).
+ * The first copy sits at the origin with theta 0, the second at (dx, dy, theta).
+ * Restated on convex polygons: the zone's boundary is the set of hull edge parts not
+ * inside another hull; the distance is the smallest distance to those parts.
+ * ------------------------------------------------------------------------- */
+#define OR_HV 32   /* vertices of a hull of up to 16 + 16 points */
+
+/* CCW convex hull of n points (Andrew's monotone chain; collinear points dropped). */
+static int hull_ccw(d2 *p, int n, d2 *h) {
+    int i, j, k = 0;
+    for (i = 1; i < n; ++i) {      /* insertion sort by (x, y) */
+        d2 v = p[i];
+        j = i - 1;
+        while (j >= 0 && (p[j].x > v.x || (p[j].x == v.x && p[j].y > v.y))) { p[j + 1] = p[j]; --j; }
+        p[j + 1] = v;
+    }
+    /* drop repeated points: a repeat of the rightmost point would open the upper chain
+       with a zero-length edge */
+    for (i = 1, j = 1; i < n; ++i)
+        if (!(fabs(p[i].x - p[j - 1].x) <= 1e-13 && fabs(p[i].y - p[j - 1].y) <= 1e-13)) p[j++] = p[i];
+    n = n > 0 ? j : 0;
+    for (i = 0; i < n; ++i) {
+        while (k >= 2 && cross3(h[k - 2], h[k - 1], p[i]) <= 0.0) --k;
+        h[k++] = p[i];
+    }
+    for (i = n - 2, j = k + 1; i >= 0; --i) {
+        while (k >= j && cross3(h[k - 2], h[k - 1], p[i]) <= 0.0) --k;
+        h[k++] = p[i];
+    }
+    return k > 1 ? k - 1 : k;
+}
+
+/* Hulls of piece_i - rot(theta) piece_j for every piece pair; returns their count. */
+int or_minkowski_hulls(const or_geom *g, double theta, double *hull_xy /* [P*P][OR_HV][2] */,
+                       int *hull_n) {
+    double c = cos(theta), s = sin(theta);
+    int i, j, a, b, k = 0;
+    for (i = 0; i < g->P; ++i)
+        for (j = 0; j < g->P; ++j, ++k) {
+            d2 pts[OR_MAXV * OR_MAXV], h[OR_MAXV * OR_MAXV + 1];
+            int n = 0;
+            for (a = 0; a < g->nverts[i]; ++a) {
+                const double *pa = g->piece_xy + ((size_t)i * g->V + a) * 2;
+                for (b = 0; b < g->nverts[j]; ++b) {
+                    const double *pb = g->piece_xy + ((size_t)j * g->V + b) * 2;
+                    double bx = c * pb[0] - s * pb[1], by = s * pb[0] + c * pb[1];
+                    pts[n++] = mk(pa[0] - bx, pa[1] - by);
+                }
+            }
+            int m = hull_ccw(pts, n, h);
+            hull_n[k] = m;
+            for (a = 0; a < m; ++a) {
+                hull_xy[((size_t)k * OR_HV + a) * 2] = h[a].x;
+                hull_xy[((size_t)k * OR_HV + a) * 2 + 1] = h[a].y;
+            }
+        }
+    return k;
+}
+
+static d2 hv(const double *hull_xy, int k, int a) {
+    return mk(hull_xy[((size_t)k * OR_HV + a) * 2], hull_xy[((size_t)k * OR_HV + a) * 2 + 1]);
+}
+
+/* strictly inside hull k (shapely's contains: the boundary is not inside) */
+static int hull_contains(const double *H, const int *hn, int k, d2 p) {
+    int a, m = hn[k];
+    if (m < 3) return 0;
+    for (a = 0; a < m; ++a)
+        if (!(cross3(hv(H, k, a), hv(H, k, (a + 1) % m), p) > 0.0)) return 0;
+    return 1;
+}
+
+static double seg_dist(d2 p, d2 u, d2 v) {
+    double ex = v.x - u.x, ey = v.y - u.y, L = ex * ex + ey * ey;
+    double t = L > 0.0 ? ((p.x - u.x) * ex + (p.y - u.y) * ey) / L : 0.0;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    double dx = p.x - (u.x + t * ex), dy = p.y - (u.y + t * ey);
+    return sqrt(dx * dx + dy * dy);
+}
+
+/* Open parameter interval (lo, hi) of u + s (v - u) strictly inside hull l. */
+static int seg_in_hull(const double *H, const int *hn, int l, d2 u, d2 v, double *lo, double *hi) {
+    int a, m = hn[l];
+    double L = -1e300, U = 1e300;
+    if (m < 3) return 0;
+    for (a = 0; a < m; ++a) {
+        d2 w0 = hv(H, l, a), w1 = hv(H, l, (a + 1) % m);
+        double f0 = cross3(w0, w1, u), f1 = cross3(w0, w1, v);   /* > 0 inside, linear in s */
+        double df = f1 - f0;
+        if (df == 0.0) {
+            if (!(f0 > 0.0)) return 0;
+        } else {
+            double r = -f0 / df;
+            if (df > 0.0) { if (r > L) L = r; } else { if (r < U) U = r; }
+        }
+    }
+    if (!(L < U)) return 0;
+    *lo = L;
+    *hi = U;
+    return 1;
+}
+
+/* Distance from p to the union's boundary: hull edge parts outside every other hull. */
+static double union_boundary_dist(const double *H, const int *hn, int nh, d2 p) {
+    double best = 1e300;
+    int k, a, l, q, r;
+    for (k = 0; k < nh; ++k) {
+        int m = hn[k];
+        if (m < 3) continue;
+        for (a = 0; a < m; ++a) {
+            d2 u = hv(H, k, a), v = hv(H, k, (a + 1) % m);
+            double cut_lo[OR_MAXP * OR_MAXP], cut_hi[OR_MAXP * OR_MAXP];
+            int nc = 0;
+            for (l = 0; l < nh; ++l) {
+                double lo, hi;
+                if (l == k) continue;
+                if (seg_in_hull(H, hn, l, u, v, &lo, &hi) && hi > 0.0 && lo < 1.0) {
+                    cut_lo[nc] = lo;   /* kept unclamped: a cut past an end covers that end */
+                    cut_hi[nc] = hi;
+                    ++nc;
+                }
+            }
+            /* sort the cuts by start, then walk the uncovered pieces of [0, 1] */
+            for (q = 1; q < nc; ++q)
+                for (r = q; r > 0 && cut_lo[r - 1] > cut_lo[r]; --r) {
+                    double t0 = cut_lo[r], t1 = cut_hi[r];
+                    cut_lo[r] = cut_lo[r - 1]; cut_hi[r] = cut_hi[r - 1];
+                    cut_lo[r - 1] = t0; cut_hi[r - 1] = t1;
+                }
+            /* uncovered pieces [s0, s1] of [0, 1]; the ends of the open cuts are not
+               covered, so a piece may be a single point */
+            double s0 = 0.0;
+            for (q = 0; q <= nc && s0 <= 1.0; ++q) {
+                double s1 = q < nc ? (cut_lo[q] < 1.0 ? cut_lo[q] : 1.0) : 1.0;
+                if (q < nc ? s1 > s0 : s1 >= s0) {
+                    d2 a0 = mk(u.x + s0 * (v.x - u.x), u.y + s0 * (v.y - u.y));
+                    d2 a1 = mk(u.x + s1 * (v.x - u.x), u.y + s1 * (v.y - u.y));
+                    double dd = seg_dist(p, a0, a1);
+                    if (dd < best) best = dd;
+                }
+                if (q < nc && cut_hi[q] > s0) s0 = cut_hi[q];
+            }
+        }
+    }
+    return best;
+}
+
+double or_separation(const or_geom *g, double dx, double dy, double theta) {
+    double H[OR_MAXP * OR_MAXP * OR_HV * 2];
+    int hn[OR_MAXP * OR_MAXP], k;
+    int nh = or_minkowski_hulls(g, theta, H, hn);
+    d2 p = mk(dx, dy);
+    int inside = 0;
+    for (k = 0; k < nh && !inside; ++k) inside = hull_contains(H, hn, k, p);
+    double d = union_boundary_dist(H, hn, nh, p);
+    return inside ? d : -d;
+}
+
+/* out[(ix * ny + iy) * nt + it] = float32 separation at (X[ix], Y[iy], T[it]). */
+void or_separation_grid(const or_geom *g, const double *X, int nx, const double *Y, int ny,
+                        const double *T, int nt, float *out, int n_threads) {
+    long long q, total = (long long)nx * ny * nt;
+#ifdef _OPENMP
+    #pragma omp parallel for schedule(dynamic, 64) num_threads(n_threads > 0 ? n_threads : 1)
+#endif
+    for (q = 0; q < total; ++q) {
+        int it = (int)(q % nt);
+        long long r = q / nt;
+        int iy = (int)(r % ny), ix = (int)(r / ny);
+        out[q] = (float)or_separation(g, X[ix], Y[iy], T[it]);
+    }
+    (void)n_threads;
+}

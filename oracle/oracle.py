@@ -57,6 +57,13 @@ def lib():
         L.or_relax.argtypes = [C.c_void_p, C.c_void_p, dp, dp, C.c_int, C.c_int, C.c_int,
                                C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                C.c_double, C.c_double, C.c_double, C.c_int, ip, C.c_int]
+        L.or_separation.restype = C.c_double
+        L.or_separation.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.or_separation_grid.restype = None
+        L.or_separation_grid.argtypes = [C.c_void_p, dp, C.c_int, dp, C.c_int, dp, C.c_int,
+                                         C.POINTER(C.c_float), C.c_int]
+        L.or_minkowski_hulls.restype = C.c_int
+        L.or_minkowski_hulls.argtypes = [C.c_void_p, C.c_double, dp, ip]
         FN = C.CFUNCTYPE(None, C.c_void_p, dp, C.c_int, C.c_int, dp, dp)
         L._FN = FN
         L.or_lbfgs.restype = C.c_int
@@ -161,6 +168,28 @@ class OracleLut:
         lib().or_lut_overlap(C.byref(self.s), radius, _dp(xyt), E, N, _dp(cost),
                              None if g is None else _dp(g))
         return cost, g
+
+
+def separation(og: OracleGeometry, dx, dy, theta):
+    """Signed separation distance of the polygon at (dx, dy, theta) from the polygon at
+    the origin (core/pack_minkowski.py:210-266): > 0 penetration depth, < 0 clearance."""
+    return lib().or_separation(og.ptr, float(dx), float(dy), float(theta))
+
+
+def separation_grid(og: OracleGeometry, X, Y, T, n_threads=8):
+    X, Y, T = (np.ascontiguousarray(a, dtype=np.float64) for a in (X, Y, T))
+    out = np.zeros((len(X), len(Y), len(T)), dtype=np.float32)
+    lib().or_separation_grid(og.ptr, _dp(X), len(X), _dp(Y), len(Y), _dp(T), len(T),
+                             out.ctypes.data_as(C.POINTER(C.c_float)), n_threads)
+    return out
+
+
+def minkowski_hulls(og: OracleGeometry, theta):
+    P = og.nv.shape[0]
+    H = np.zeros((P * P, 32, 2))
+    n = np.zeros(P * P, dtype=np.int32)
+    lib().or_minkowski_hulls(og.ptr, float(theta), _dp(H), n.ctypes.data_as(C.POINTER(C.c_int)))
+    return [H[k, :n[k]] for k in range(P * P)]
 
 
 def hungarian_f32(cost):

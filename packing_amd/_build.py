@@ -19,6 +19,7 @@ SOURCES = [
     "pk_feasible.hip",
     "pk_lap.hip",
     "pk_lbfgs.hip",
+    "pk_sep.hip",
 ]
 
 ARCH = os.environ.get("PK_OFFLOAD_ARCH", "gfx950")
@@ -36,9 +37,15 @@ FLAGS = [
 ]
 
 
+# Per-file additions.  The separation builder decides containment by the signs of exact
+# cross products (pieces of a decomposition share edges and vertices, so many are exactly
+# zero); fused multiply-adds would turn those zeros into rounding noise.
+EXTRA = {"pk_sep.hip": ["-ffp-contract=off"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{p.stderr}")

@@ -41,6 +41,7 @@ _SIGS = {
                             C.c_double, C.c_double, i32, i32, C.POINTER(vp)]),
     "pk_lut_destroy": (i32, [vp]),
     "pk_lut_build": (i32, [vp, vp, i32, vp, i32, vp, i32, vp, vp]),
+    "pk_sep_build": (i32, [vp, vp, i32, vp, i32, vp, i32, vp, vp]),
     "pk_lut_overlap": (i32, [vp, vp, vp, vp, i32, i32, vp, vp, i32, vp, vp]),
     "pk_relax_step": (i32, [vp, vp, vp]),
     "pk_relax_run_noalloc": (i32, [vp, vp, i32, vp]),

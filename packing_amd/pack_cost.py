@@ -124,7 +124,8 @@ class CollisionCost(Cost):
     def _ensure_lut_initialized(self):
         if self.use_lookup_table and self._lut is None:
             if self.use_separation:
-                raise NotImplementedError("tables are built for the overlap area only")
+                raise NotImplementedError("tables are built for the overlap area and the exact separation "
+                                          "(CollisionCostExactSeparation) only")
             # the reference tabulates the x >= 0 half (core/pack_cuda_lut.py:128-131), which
             # holds for mirror-symmetric polygons only; others get the full x range
             from .context import get_geometry
@@ -189,3 +190,53 @@ class CollisionCostOverlappingArea(CollisionCost):
 class CollisionCostSeparation(CollisionCost):
     """Summed squared SAT separation (core/pack_cost.py:821-1116)."""
     use_separation = True
+
+
+@dataclass
+class CollisionCostExactSeparation(CollisionCost):
+    """Collision cost from the exact Minkowski separation distance, table only
+    (core/pack_cost.py:1119-1247): per pair max(0, sep)^2, sep the signed separation of
+    the two polygons (penetration depth positive).  The table is computed on the GPU in
+    double (pk_sep_build; the reference uses shapely on the CPU) on the reference's grid
+    and cached on disk in the binary format of lut_cache, keyed by the grid parameters
+    and the polygon's fingerprint (the reference keys its pickle by an md5 of the tree
+    vertices, :1131-1142)."""
+    use_separation = False
+
+    def _cache_key(self) -> str:
+        from .context import get_geometry
+        return (f"exact_sep_Nx{self.lut_N_x}_Ny{self.lut_N_y}_Nt{self.lut_N_theta}_trim{self.lut_trim_zeros}"
+                f"_tree{get_geometry().fingerprint()[:8]}")
+
+    def _ensure_lut_initialized(self):
+        if not self.use_lookup_table:
+            raise ValueError(f"{self.__class__.__name__} requires use_lookup_table=True")
+        if self._lut is not None:
+            return
+        import os
+        from . import lut_cache
+        from .context import get_geometry
+        key = self._cache_key()
+        path = os.path.join(lut_cache.cache_dir(), key + ".pklut")
+        hit = lut_cache.load(path, key)
+        if hit is not None:
+            X, Y, T, vals, quad, mirror = hit
+            self._lut = pack_cuda_lut.LookupTable(X=X, Y=Y, theta=T, vals=vals, apply_quadratic_transform=quad,
+                                                  mirror_x=mirror)
+            return
+        full_x = not get_geometry().mirror_symmetric
+        lut = pack_cuda_lut.LookupTable.build_separation(self.lut_N_x, self.lut_N_y, self.lut_N_theta,
+                                                         full_x=full_x, trim_zeros=self.lut_trim_zeros)
+        try:
+            lut_cache.save(path, key, lut)
+        except OSError:
+            pass   # an unwritable cache directory only costs a rebuild next time
+        self._lut = lut
+
+    def _internal(self, sol, cost, grad_xyt, evaluate_gradient, crystal_axes=None, only_self=False):
+        raise NotImplementedError(f"{self.__class__.__name__} has no exact kernel; use_lookup_table=True")
+
+    def _compute_cost(self, sol, cost, grad_xyt, grad_bound, evaluate_gradient):
+        assert self.use_lookup_table, f"{self.__class__.__name__} requires use_lookup_table=True"
+        assert not sol.periodic, f"{self.__class__.__name__} does not support periodic boundaries"
+        super()._compute_cost(sol, cost, grad_xyt, grad_bound, evaluate_gradient)

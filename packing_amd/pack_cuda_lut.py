@@ -129,6 +129,26 @@ class LookupTable:
         return cls(X=X, Y=Y, theta=T, vals=vals, mirror_x=not full_x)
 
     @classmethod
+    def build_separation(cls, N_x=900, N_y=900, N_theta=900, full_x=True, trim_zeros=True, device="cuda",
+                         verbose=False) -> "LookupTable":
+        """Signed separation-distance table (positive = penetration depth, negative =
+        clearance) computed on the GPU in double by pk_sep_build, with the quadratic
+        transform max(0, v)^2 applied per pair: the table of
+        CollisionCostExactSeparation (core/pack_cost.py:1170-1207)."""
+        X, Y, T = cls.default_grid(N_x, N_y, N_theta, full_x)
+        dev = torch.device(device)
+        Xt, Yt, Tt = (torch.tensor(a, device=dev) for a in (X, Y, T))
+        vals = torch.empty(len(X), len(Y), len(T), dtype=torch.float32, device=dev)
+        dg = device_geometry(dev)
+        N.check(N.lib().pk_sep_build(dg.handle, Xt.data_ptr(), len(X), Yt.data_ptr(), len(Y),
+                                     Tt.data_ptr(), len(T), vals.data_ptr(),
+                                     N.stream_handle(None)), "pk_sep_build")
+        if trim_zeros:
+            X, Y, T, vals = cls._trim_zero_edges(X, Y, T, vals, verbose=verbose)
+            vals = vals.contiguous()
+        return cls(X=X, Y=Y, theta=T, vals=vals, apply_quadratic_transform=True, mirror_x=not full_x)
+
+    @classmethod
     def build_from_function(cls, eval_fn, X: Optional[np.ndarray] = None, Y: Optional[np.ndarray] = None,
                             theta: Optional[np.ndarray] = None, N_x: int = 400, N_y: int = 400,
                             N_theta: int = 400, trim_zeros: bool = True, verbose: bool = True):
