@@ -315,6 +315,22 @@ int pk_lap_hungarian(int dtype, const void* cost, int B, int N, int32_t* row_mat
 int pk_diversity(const float* pop, const float* ref, int n_pop, int n_ref, int n_trees, float cos_a,
                  float sin_a, int mirror, float* out, void* stream);
 
+/*
+ * Genetic-diversity matrix in one launch: out[t][p][r] (transform t < 8 if transform, else
+ * 1) = assignment cost between population individual p and reference individual r after
+ * symmetry transform t of the reference (rotations by 0, pi/2, pi, 3 pi/2, then the same
+ * after mirroring y and theta -> pi - theta), with tree-to-tree distance
+ * sqrt(dx^2 + dy^2 + wrapped dtheta^2).  algorithm 0 = Hungarian (total = float32 row-order
+ * sum over the optimal assignment), 1 = sum over rows of the row minimum, 2 = sum over
+ * columns of the column minimum.  The caller takes the minimum over t.  Replaces
+ * compute_genetic_diversity_matrix's cost-tensor build and solve_lap_batch call
+ * (core/kaggle_support.py:2014-2133, core/lap_batch.py:457-534): the N x N matrix of each
+ * problem is built in LDS and solved there, with no (8, N_pop, N, N_ref, N) tensor.
+ * pop (n_pop, n_trees, 3), ref (n_ref, n_trees, 3) float32; n_trees <= 200.
+ */
+int pk_diversity_matrix(const float* pop, const float* ref, int n_pop, int n_ref, int n_trees, int transform,
+                        int algorithm, float* out, void* stream);
+
 /* sizeof the per-slot L-BFGS record of pk_relax_step (PK_LBFGS_SLOT_BYTES). */
 int pk_relax_slot_bytes(void);
 

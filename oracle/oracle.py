@@ -582,3 +582,52 @@ def lbfgs_wolfe(func, x0, lr=1.0, max_iter=20, max_eval=None, tolerance_grad=1e-
         if stop_on_cost_increase:
             active &= ~((loss > prev_loss0) & active)
     return x
+
+
+# ---------------------------------------------------------------------------------
+# Genetic-diversity matrix (core/kaggle_support.py:2014-2133): per symmetry transform of
+# the reference individuals, the N x N tree distance matrices (double, numpy's
+# remainder, rounded to float32 once), their assignment costs (the float32 Hungarian
+# restatement, or min-cost row / column sums accumulated in row / column order), and the
+# minimum over transforms.
+# ---------------------------------------------------------------------------------
+def diversity_cost_matrices(pop, ref, transform=True):
+    """(T, N_pop, N_ref, N, N) float32 cost matrices."""
+    pop = np.asarray(pop, dtype=np.float32).astype(np.float64)
+    ref = np.asarray(ref, dtype=np.float32).astype(np.float64)
+    trs = [(0.0, False), (np.pi / 2, False), (np.pi, False), (3 * np.pi / 2, False),
+           (0.0, True), (np.pi / 2, True), (np.pi, True), (3 * np.pi / 2, True)]
+    if not transform:
+        trs = trs[:1]
+    out = []
+    for rot, mir in trs:
+        x, y, th = ref[..., 0].copy(), ref[..., 1].copy(), ref[..., 2].copy()
+        if mir:
+            y = -y
+            th = np.pi - th
+        if rot != 0.0:
+            c, s = np.cos(rot), np.sin(rot)
+            x, y = x * c - y * s, x * s + y * c
+            th = th + rot
+        th = np.remainder(th + np.pi, 2 * np.pi) - np.pi
+        dx = pop[:, None, :, None, 0] - x[None, :, None, :]
+        dy = pop[:, None, :, None, 1] - y[None, :, None, :]
+        dt = np.remainder((pop[:, None, :, None, 2] - th[None, :, None, :]) + np.pi, 2 * np.pi) - np.pi
+        out.append(np.sqrt(dx * dx + dy * dy + dt * dt).astype(np.float32))
+    return np.stack(out)
+
+
+def diversity_matrix(pop, ref, algorithm="hungarian", transform=True):
+    """(T, N_pop, N_ref) per-transform assignment costs and their minimum (N_pop, N_ref)."""
+    C = diversity_cost_matrices(pop, ref, transform)
+    T, P, R, N, _ = C.shape
+    flat = C.reshape(-1, N, N)
+    if algorithm == "hungarian":
+        _, tot = hungarian_f32(flat)
+    else:
+        m = flat.min(axis=2) if algorithm == "min_cost_row" else flat.min(axis=1)
+        tot = np.zeros(len(flat), dtype=np.float32)
+        for k in range(N):             # float32, sequential in row (column) order
+            tot = (tot + m[:, k]).astype(np.float32)
+    per = tot.reshape(T, P, R)
+    return per, per.min(axis=0)

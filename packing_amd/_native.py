@@ -61,6 +61,7 @@ _SIGS = {
     "pk_lbfgs_ls_end": (i32, [vp, vp, vp]),
     "pk_lap_hungarian": (i32, [i32, vp, i32, i32, vp, vp, vp, vp]),
     "pk_diversity": (i32, [vp, vp, i32, i32, i32, C.c_float, C.c_float, i32, vp, vp]),
+    "pk_diversity_matrix": (i32, [vp, vp, i32, i32, i32, i32, i32, vp, vp]),
 }
 
 

@@ -80,3 +80,41 @@ def test_diversity_vs_oracle(oracle_mod, mirror):
                                             np.cos(a), np.sin(a), mirror).cpu().numpy()
     want = oracle_mod.diversity(pop, ref, np.float32(np.cos(a)), np.float32(np.sin(a)), mirror)
     np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("algorithm", ["hungarian", "min_cost_row", "min_cost_col"])
+@pytest.mark.parametrize("n,n_pop,n_ref", [(15, 5, 4), (50, 3, 3), (200, 1, 2)])
+def test_diversity_matrix_vs_oracle(oracle_mod, algorithm, n, n_pop, n_ref):
+    """compute_genetic_diversity_matrix in one launch (cost matrices built in LDS and
+    solved there) equals the oracle bit for bit, per transform and after the minimum."""
+    from packing_amd.diversity import compute_genetic_diversity_matrix
+    from packing_amd.lap_batch import LAPConfig
+    rng = np.random.default_rng(n)
+    L = np.sqrt(n)
+    pop = np.concatenate([rng.uniform(0, L, (n_pop, n, 2)), rng.uniform(0, 2 * np.pi, (n_pop, n, 1))], 2)
+    ref = np.concatenate([rng.uniform(0, L, (n_ref, n, 2)), rng.uniform(0, 2 * np.pi, (n_ref, n, 1))], 2)
+    pop, ref = pop.astype(np.float32), ref.astype(np.float32)
+    best, per = compute_genetic_diversity_matrix(torch.tensor(pop, device="cuda"), torch.tensor(ref, device="cuda"),
+                                                 LAPConfig(algorithm=algorithm), allow_shortcut=False,
+                                                 per_transform=True)
+    per_o, best_o = oracle_mod.diversity_matrix(pop, ref, algorithm)
+    assert np.array_equal(per.cpu().numpy(), per_o)
+    assert np.array_equal(best.cpu().numpy(), best_o)
+
+
+def test_diversity_single_reference_and_shortcut():
+    """compute_genetic_diversity (one reference) is the matrix's column; the shortcut path
+    (min_cost_row with allow_shortcut) agrees with the full min_cost_row path to float32
+    rounding of its float32 formula."""
+    from packing_amd.diversity import compute_genetic_diversity, compute_genetic_diversity_matrix
+    from packing_amd.lap_batch import LAPConfig
+    rng = np.random.default_rng(3)
+    pop = torch.tensor(rng.uniform(0, 5, (6, 25, 3)), device="cuda", dtype=torch.float32)
+    ref = torch.tensor(rng.uniform(0, 5, (3, 25, 3)), device="cuda", dtype=torch.float32)
+    m = compute_genetic_diversity_matrix(pop, ref)
+    for r in range(3):
+        assert torch.equal(compute_genetic_diversity(pop, ref[r]), m[:, r])
+    cfg = LAPConfig(algorithm="min_cost_row")
+    a = compute_genetic_diversity_matrix(pop, ref, cfg)
+    b = compute_genetic_diversity_matrix(pop, ref, cfg, allow_shortcut=False)
+    assert torch.allclose(a, b, rtol=1e-4, atol=1e-4)
