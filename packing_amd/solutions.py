@@ -75,6 +75,26 @@ class SolutionCollectionSquare(SolutionCollection):
         z = torch.zeros_like(self.h[:, 0])
         return torch.stack([self.h[:, 0], z, z, self.h[:, 0]], -1)
 
+    def snap(self):
+        """h = the smallest square holding every tree's outline, trees recentred on it
+        (core/kaggle_support.py:824-880); nothing with a fixed h."""
+        if self.use_fixed_h:
+            return
+        from .context import get_geometry
+        v = torch.tensor(get_geometry().outline, dtype=self.xyt.dtype, device=self.xyt.device)
+        c, s = torch.cos(self.xyt[..., 2:3]), torch.sin(self.xyt[..., 2:3])
+        vx = (c * v[:, 0] - s * v[:, 1] + self.xyt[..., 0:1]).reshape(self.xyt.shape[0], -1)
+        vy = (s * v[:, 0] + c * v[:, 1] + self.xyt[..., 1:2]).reshape(self.xyt.shape[0], -1)
+        x0, x1 = vx.min(1).values, vx.max(1).values
+        y0, y1 = vy.min(1).values, vy.max(1).values
+        size = torch.maximum(x1 - x0, y1 - y0)
+        self.xyt[..., 0] -= ((x0 + x1) / 2.0)[:, None]
+        self.xyt[..., 1] -= ((y0 + y1) / 2.0)[:, None]
+        if self.h is None:
+            self.h = torch.stack([size, 0 * size, 0 * size], 1)
+        else:
+            self.h = torch.stack([torch.minimum(size, self.h[:, 0]), 0 * size, 0 * size], 1)
+
 
 class SolutionCollectionSquareSymmetric180(SolutionCollectionSquare):
     """Two images (x, y, t) and (-x, -y, t + pi) per genotype tree
