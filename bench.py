@@ -86,24 +86,36 @@ def overlap_kernel_roofline(geom, dev, n_ens=1024, n_trees=100, reps=10, seed=1)
             "frac_of_nonpacked_fp32": round(tf / 78.6, 4)}
 
 
-def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0):
+def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0, n_sat=4096):
     """Continuous-batching server under Poisson arrivals (rate per second, N log-uniform
-    in [10, 200], iters L-BFGS iterations per request) and the plain per-request path on
-    the first n_plain requests of the same stream."""
+    in [10, 200], iters L-BFGS iterations per request), the same server at saturation
+    (n_sat requests all available at the start), and the plain per-request path on the
+    first n_plain requests of the Poisson stream."""
     from packing_amd.relax import RelaxOptions
     from packing_amd.serving import RelaxServer, poisson_requests, serve_plain
     opts = RelaxOptions(mode="lut", lut=lut, max_iter=iters, history=8, max_step=0.1)
     srv = RelaxServer(opts, n_slots=4096, n_max=200, hbm_budget_bytes=64e9, chunk=100, device=dev,
                       geometry=geom)
-    warm = poisson_requests(8, rate=rate, seed_arrivals=1, seed_requests=2)
+    warm = poisson_requests(64, rate=rate, seed_arrivals=1, seed_requests=2)
     srv.serve(warm, realtime=False)
     reqs = poisson_requests(n_req, rate=rate)
     rep = srv.serve(reqs)
+    ms = lambda v: round(v * 1e3, 3)
     out = {"requests": rep["requests"], "rate_per_s": rate, "iters": iters,
-           "throughput": round(rep["throughput"], 2), "p50_ms": round(rep["p50_s"] * 1e3, 3),
-           "p99_ms": round(rep["p99_s"] * 1e3, 3), "launches": rep["launches"],
+           "throughput": round(rep["throughput"], 2), "p50_ms": ms(rep["p50_s"]), "p99_ms": ms(rep["p99_s"]),
+           # latency = arrival -> result in host memory: queue (arrival -> admission with
+           # the host-to-device copy) + service (admission -> result back on the host)
+           "queue_p50_ms": ms(rep["queue_p50_s"]), "queue_p99_ms": ms(rep["queue_p99_s"]),
+           "service_p50_ms": ms(rep["service_p50_s"]), "service_p99_ms": ms(rep["service_p99_s"]),
+           "launch_steps": rep["launches"], "steps_in_flight": srv.in_flight,
            "feasible_fraction_fp64": rep["feasible_fraction"], "slots": rep["slots"],
-           "hbm_budget_gb": 64}
+           "slot_table_gb": round(rep["slot_table_bytes"] / 1e9, 3), "hbm_budget_gb": 64}
+    # saturation: every request available at the start, so throughput is the server's capacity
+    sat = poisson_requests(n_sat, rate=rate, seed_arrivals=8, seed_requests=9)
+    rs = srv.serve(sat, realtime=False)
+    out["saturation"] = {"requests": rs["requests"], "throughput": round(rs["throughput"], 2),
+                         "service_p50_ms": ms(rs["service_p50_s"]), "service_p99_ms": ms(rs["service_p99_s"]),
+                         "feasible_fraction_fp64": rs["feasible_fraction"]}
     if n_plain > 0:
         pr = poisson_requests(n_req, rate=rate)[:n_plain]
         pl = serve_plain(pr, opts, geometry=geom, device=dev)
@@ -112,7 +124,7 @@ def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0):
                         "p50_ms": round(pl["p50_s"] * 1e3, 3), "p99_ms": round(pl["p99_s"] * 1e3, 3),
                         "service_p50_ms": round(float(np.percentile(svc, 50)) * 1e3, 3),
                         "feasible_fraction_fp64": pl["feasible_fraction"]}
-        same = all(np.array_equal(a.result.cpu().numpy(), b.result.cpu().numpy())
+        same = all(np.array_equal(a.result.numpy(), b.result.numpy())
                    for a, b in zip(pr, reqs[:n_plain]))
         out["plain"]["bitwise_equal_to_served"] = bool(same)
         out["throughput_vs_plain"] = round(out["throughput"] / max(pl["throughput"], 1e-12), 2)
@@ -182,9 +194,14 @@ def bucket_roofline(b, sizes, opts, chunk, n_chunks, stream):
                 per[k]["lookups"] += int(b.stats[1].item())
                 per[k]["bytes"] += float(np.sum((n_state[idx] * 4 * (4 + 2 * rows))[run]))
                 rec0[idx] = rec1[idx]
+    from packing_amd.autotune import choose_block
     for k, q in enumerate(per):
         q["bytes"] += q["lookups"] * 32.0
-        q["kernel"] = f"pk::k_relax_w<2, {-(-(3 * q['n_max'] + 3) // 64)}, 8>"
+        q["block"] = b._block_for(q["n_max"])
+        q["tile_source"] = "tiles.json" if choose_block(opts.mode, q["n_max"], b.geometry.n_pieces,
+                                                        opts.history) else "library rule"
+        q["kernel"] = f"pk::k_relax_w<2, {-(-(3 * q['n_max'] + 3) // 64)}, 8>" if q["block"] in (0, 1) else \
+            f"pk::k_relax<2, {q['block']}, ...>"
     dom = max(per, key=lambda q: sum(q["ms"]))
     avg_s = float(np.mean(dom["ms"])) / 1e3
     alg = dom["bytes"] / n_chunks
@@ -202,10 +219,110 @@ def bucket_roofline(b, sizes, opts, chunk, n_chunks, stream):
             "avg_launch_ms": round(avg_s * 1e3, 3), "alg_bytes_per_launch": alg,
             "lookups_per_launch": dom["lookups"] / n_chunks,
             "buckets": [{"kernel": q["kernel"], "instances": q["instances"], "n_max": q["n_max"],
+                         "block": q["block"], "tile_source": q["tile_source"],
                          "avg_launch_ms": round(float(np.mean(q["ms"])), 3),
                          "alg_bytes_per_launch": q["bytes"] / n_chunks,
                          "achieved_GBps": round(q["bytes"] / n_chunks / (float(np.mean(q["ms"])) / 1e3) / 1e9, 2)}
                         for q in per]}
+
+
+def c1_plain_section(dev, iters=500):
+    """Plain method on the GPU configuration: 1024 instances x 100 trees, exact overlap
+    area + boundary, fixed square boxes, 500 L-BFGS iterations (history 8, max_step 0.1),
+    from the same initial poses through the reference API (pack_dynamics.OptimizerBFGS):
+    the reference loop (one pack_cost evaluation + the drop-in batched L-BFGS kernels per
+    iteration, engine="reference") and the fused engine (engine="auto")."""
+    import torch
+    from packing_amd import pack_cost as PC
+    from packing_amd.feasibility import overlapping_pairs
+    from packing_amd.pack_dynamics import OptimizerBFGS
+    from packing_amd.solutions import SolutionCollectionSquare
+    from packing_amd.workloads import uniform_workload
+    x, h = uniform_workload(1024, 100, seed=2)
+    cost = PC.CostCompound(costs=[PC.BoundaryDistanceCost(), PC.CollisionCostOverlappingArea()])
+
+    def sol():
+        return SolutionCollectionSquare(torch.tensor(x, device=dev), torch.tensor(h, device=dev), use_fixed_h=True)
+
+    out = {"workload": "1024 x 100 trees, exact area + boundary, 500 iterations, history 8, FP32"}
+    for engine in ("reference", "auto"):
+        OptimizerBFGS(cost=cost, n_iterations=3, history_size=8, max_step=0.1).run_simulation(sol(), engine=engine)
+        torch.cuda.synchronize()
+        opt = OptimizerBFGS(cost=cost, n_iterations=iters, history_size=8, max_step=0.1)
+        s0 = sol()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = opt.run_simulation(s0, engine=engine)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        pairs = overlapping_pairs(r.xyt).cpu().numpy()
+        out[engine] = {"value": round(1024 / t, 2), "unit": "relaxations/s", "seconds": round(t, 4),
+                       "feasible_fraction_fp64": float((pairs == 0).mean())}
+    out["fused_vs_reference_loop"] = round(out["auto"]["value"] / out["reference"]["value"], 2)
+    return out
+
+
+def c0_section(geom, dev, budget_s, iters=200):
+    """The CPU configuration: 16 instances x 20 trees in FP64, exact overlap area +
+    boundary, 200 L-BFGS iterations (history 8): the FP64 oracle on the host cores (OpenMP
+    over instances) and the drop-in path on the GPU in FP64 (pack_dynamics reference loop)."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from packing_amd import pack_cost as PC
+    from packing_amd.pack_dynamics import OptimizerBFGS
+    from packing_amd.solutions import SolutionCollectionSquare
+    from packing_amd.workloads import uniform_workload
+    x, h = uniform_workload(16, 20, seed=1, dtype=np.float64)
+    cores = len(os.sched_getaffinity(0))
+    og = O.OracleGeometry(geom)
+    reps, t_cpu = 0, 0.0
+    while t_cpu < budget_s and reps < 50:
+        t0 = time.perf_counter()
+        O.relax(og, x, h, max_iter=iters, m=8, max_step=0.1, n_threads=cores)
+        t_cpu += time.perf_counter() - t0
+        reps += 1
+    cost = PC.CostCompound(costs=[PC.BoundaryDistanceCost(), PC.CollisionCostOverlappingArea()])
+    opt = OptimizerBFGS(cost=cost, n_iterations=iters, history_size=8, max_step=0.1)
+
+    def sol():
+        return SolutionCollectionSquare(torch.tensor(x, device=dev), torch.tensor(h, device=dev), use_fixed_h=True)
+
+    opt.run_simulation(sol())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    opt.run_simulation(sol())
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    return {"workload": "16 x 20 trees, FP64, exact area + boundary, 200 iterations, history 8",
+            "cpu": {"value": round(16 * reps / t_cpu, 2), "unit": "relaxations/s", "cores": cores,
+                    "kind": "port", "sample": f"{reps} runs of the whole configuration, {t_cpu:.1f} s"},
+            "gpu_reference_loop_fp64": {"value": round(16 / t_gpu, 2), "unit": "relaxations/s",
+                                        "seconds": round(t_gpu, 4)}}
+
+
+def infeasible_breakdown(b, sizes, pairs, lut):
+    """The final packings that still overlap in FP64: how many the table sees as free (zero
+    table cost: the interpolated table, not the exact polygon) and their stop reasons."""
+    import torch
+    from packing_amd.pack_cuda_lut import overlap_multi_ensemble
+    bad = np.where(pairs > 0)[0]
+    rec = b.slot_records()
+    zero = 0
+    for i in bad:
+        c = torch.empty(1, device=b.device)
+        overlap_multi_ensemble(b.xyt(int(i))[None].contiguous(), c, lut)
+        zero += float(c.item()) == 0.0
+    names = {1: "grad", 2: "dir", 4: "param", 8: "loss", 16: "rel", 32: "increase", 64: "eval", 128: "iter"}
+    reasons = {}
+    for i in bad:
+        r = int(rec[i, 4])
+        key = "+".join(v for k, v in names.items() if r & k) or "none"
+        reasons[key] = reasons.get(key, 0) + 1
+    return {"overlapping": int(len(bad)), "zero_table_cost": int(zero),
+            "overlapping_pairs_total": int(pairs[bad].sum()) if len(bad) else 0,
+            "stop_reasons": reasons,
+            "mean_trees": round(float(sizes[bad].mean()), 1) if len(bad) else None}
 
 
 def cpu_baseline(geom, lut, sizes, xs, hs, opts, budget_s):
@@ -231,9 +348,11 @@ def cpu_baseline(geom, lut, sizes, xs, hs, opts, budget_s):
                     n_threads=cores, lut=OL)
         t_used += time.perf_counter() - t0
         done += len(batch)
+    part = "all instances" if done >= len(sizes) else \
+        f"the {done} instances closest to the mean size N={np.mean(sizes):.0f}"
     return {"value": done / t_used, "unit": "relaxations/s", "cores": cores, "kind": "port",
-            "sample": f"{done} instances of the workload (sizes near the mean N={np.mean(sizes):.0f}), "
-                      f"{opts.max_iter} iterations each, FP64 oracle, {t_used:.1f} s"}
+            "sample": f"{part} of the workload, {opts.max_iter} iterations each, FP64 oracle "
+                      f"(all-pairs loop within the cutoff, as the reference's table kernel), {t_used:.1f} s"}
 
 
 def main():
@@ -257,6 +376,12 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the per-bucket kernel measurement (the launches profiled with rocprofv3)")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--tune", action="store_true",
+                    help="only measure every tile on each size bucket and save packing_amd/tiles.json")
+    ap.add_argument("--tiles-out", default=None, help="where --tune writes the table")
+    ap.add_argument("--no-configs", action="store_true", help="skip the plain-method configuration lines")
+    ap.add_argument("--serve-saturation", type=int, default=4096,
+                    help="requests of the saturation session")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -306,6 +431,18 @@ def main():
     if args.roofline_only:
         print(json.dumps({"roofline": bucket_roofline(b, sizes, opts, chunk, n_chunks, stream)}))
         return
+    if args.tune:
+        # the headline workload (table mode, every size bucket) and the plain-method GPU
+        # configuration's shape in exact-area mode (1024 x 100)
+        from packing_amd.autotune import tune_batch
+        from packing_amd.workloads import uniform_workload
+        res = tune_batch(b, iters=chunk, stream=stream)
+        x1, h1 = uniform_workload(1024, 100, seed=2)
+        b1 = RelaxBatch(list(x1), h1, RelaxOptions(mode="area", max_iter=args.iters, history=8, max_step=0.1),
+                        geometry=geom, device=dev)
+        res += tune_batch(b1, iters=chunk, save=True, stream=stream, path=args.tiles_out)
+        print(json.dumps({"tiles": res}))
+        return
 
     # one graph = reset + all chunked launches of one relaxation of the batch
     with torch.cuda.stream(stream):
@@ -341,6 +478,8 @@ def main():
     feasible_frac = float((pairs == 0).mean())
     rec = b.slot_records()
     loss = b.loss.cpu().numpy()
+    infeasible = infeasible_breakdown(b, sizes, pairs, lut)
+    log(f"infeasible: {infeasible}")
 
     roofline = bucket_roofline(b, sizes, opts, chunk, n_chunks, stream)
     log(f"roofline: {roofline}")
@@ -372,8 +511,17 @@ def main():
 
     serving = None
     if args.serve_requests > 0:
-        serving = serving_section(geom, lut, dev, args.serve_requests, args.serve_plain, args.serve_iters)
+        serving = serving_section(geom, lut, dev, args.serve_requests, args.serve_plain, args.serve_iters,
+                                  n_sat=args.serve_saturation)
         log(f"serving: {serving}")
+
+    c1 = c0 = None
+    if not args.no_configs:
+        c1 = c1_plain_section(dev)
+        log(f"plain-method GPU configuration: {c1}")
+        if not args.no_cpu:
+            c0 = c0_section(geom, dev, budget_s=5.0)
+            log(f"plain-method CPU configuration: {c0}")
 
     cpu = None
     if not args.no_cpu:
@@ -402,10 +550,13 @@ def main():
         "plain": plain,
         "speedup_vs_plain": round(value / plain["value"], 2) if plain else None,
         "feasible_fraction_fp64": feasible_frac,
+        "infeasible": infeasible,
         "iterations_mean": round(float(rec[:, 0].mean()), 2),
         "stopped_before_max_iter": round(float((rec[:, 0] < args.iters).mean()), 4),
         "serving": serving,
         "overlap_kernel": ovl,
+        "plain_method_gpu_config": c1,
+        "plain_method_cpu_config": c0,
         "final_loss_median": float(np.median(loss)),
         "table_build_s": round(t_lut, 4),
         "device": info,

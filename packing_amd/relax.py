@@ -166,7 +166,7 @@ class RelaxBatch:
 
     def _auto_block(self) -> int:
         from .autotune import choose_block
-        return choose_block(self.opts.mode, self.n_tree_max, self.geometry.n_pieces)
+        return choose_block(self.opts.mode, self.n_tree_max, self.geometry.n_pieces, self.opts.history)
 
     def _launch_one(self, iters, stream, **kw) -> None:
         p = self._params(iters, **kw)
@@ -202,16 +202,17 @@ class RelaxBatch:
         """Size buckets as (first position in the size order, count, largest tree count)."""
         return list(self._buckets)
 
-    def launch_bucket(self, k: int, iters: int, stream=None) -> None:
-        """Advance only the instances of size bucket k (one kernel launch)."""
+    def launch_bucket(self, k: int, iters: int, stream=None, block: int | None = None) -> None:
+        """Advance only the instances of size bucket k (one kernel launch); block: the
+        tile (default: opts.block, else the measured tile table, else the library's rule)."""
         off, cnt, nmax = self._buckets[k]
-        blk = int(self.opts.block) or self._block_for(nmax)
+        blk = int(block) if block is not None else (int(self.opts.block) or self._block_for(nmax))
         self._launch_one(iters, stream, order=self.order[off:off + cnt], n_launch=cnt, n_tree_max=nmax,
                          block=blk)
 
     def _block_for(self, n_max: int) -> int:
         from .autotune import choose_block
-        return choose_block(self.opts.mode, n_max, self.geometry.n_pieces)
+        return choose_block(self.opts.mode, n_max, self.geometry.n_pieces, self.opts.history)
 
     # ------------------------------------------------------------------ paths
     def run(self, iters_per_launch: int | None = None, stream=None) -> None:

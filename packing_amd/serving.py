@@ -1,22 +1,29 @@
 """Continuous-batching relaxation server on one GPU.
 
-Requests (initial poses + box) arrive over time.  The server keeps a device slot table
-(fixed-capacity slots of up to ``n_max`` trees: state, gradient, L-BFGS history and
-optimizer record in HBM) and admits a request into a free slot when its memory fits
-the HBM budget.  One persistent launch advances every occupied slot by ``chunk``
-iterations (the fused engine, pk_relax_step); slots finish independently and are
-freed as soon as the host sees their status, so requests that arrive while others run
-join the next launch.  Each slot runs exactly the computation it would run alone
-(per-instance L-BFGS, fixed-point reductions), so a served result is bitwise the same
-as relaxing the request by itself.
+Requests (initial poses + box, in host memory) arrive over time.  The server keeps a
+device slot table (fixed-capacity slots of up to ``n_max`` trees: state, gradient, L-BFGS
+history and optimizer record in HBM) and admits a request into a free slot when its
+memory fits the HBM budget: its poses are copied host -> device then, inside its
+latency.  A step advances every occupied slot by ``chunk`` iterations of the fused
+engine (pk_relax_step), one launch per size class (the register rows R = ceil((3N + 3)
+/ 64) of the one-wave kernel: R <= 3, 4-5, 6-8, 9-10, and larger), each with the kernel
+for its own largest request, on forked streams that run concurrently.  Two steps are
+kept in flight, so the GPU never waits for the host to retire and admit: a request
+admitted while step k runs joins step k + 1 or k + 2.  Finished slots are retired from
+the status copy of the step that finished them; results and the FP64 feasibility check
+(pk_feasible) come back to the host on a separate stream while the next step runs, and
+a request's latency ends when its result is in host memory.
 
-Final packings are checked in FP64 on the GPU (pk_feasible) as they complete.
+Each slot runs exactly the computation it would run alone (per-instance L-BFGS,
+fixed-point reductions), so a served result is bitwise the same as relaxing the request
+by itself, whatever it was batched with.
 """
 from __future__ import annotations
 
 import ctypes as C
 import time
-from dataclasses import dataclass, field
+from collections import deque
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -35,7 +42,7 @@ class Request:
     t_admit: float = -1.0
     t_done: float = -1.0
     slot: int = -1
-    result: np.ndarray | None = None
+    result: torch.Tensor | None = None    # (N, 3) float32, host memory
     loss: float = float("nan")
     overlapping_pairs: int = -1
 
@@ -57,9 +64,32 @@ def slot_bytes(n_max: int, history: int) -> int:
     return 4 * n * (2 + 2 * history) + N.PK_LBFGS_SLOT_BYTES + 16
 
 
+# Size classes by register rows R of the one-wave kernel (upper bounds; beyond the last,
+# one class for the workgroup kernel).
+R_CLASS_MAX = (3, 5, 8, 10)
+
+
+def size_class(n_trees: int) -> int:
+    r = -(-(3 * n_trees + 3) // 64)
+    for k, hi in enumerate(R_CLASS_MAX):
+        if r <= hi:
+            return k
+    return len(R_CLASS_MAX)
+
+
+class _Step:
+    """One step in flight: its completion event, the status copy it ends with, and the
+    request of every slot it advanced."""
+    __slots__ = ("event", "status_h", "slots")
+
+    def __init__(self, event, status_h, slots):
+        self.event, self.status_h, self.slots = event, status_h, slots
+
+
 class RelaxServer:
     def __init__(self, opts: RelaxOptions, n_slots: int = 4096, n_max: int = 200,
-                 hbm_budget_bytes: float = 64e9, chunk: int = 100, device="cuda", geometry=None):
+                 hbm_budget_bytes: float = 64e9, chunk: int = 100, device="cuda", geometry=None,
+                 in_flight: int = 2):
         self.opts = opts
         self.dev = torch.device(device)
         self.geometry = geometry or get_geometry()
@@ -72,6 +102,7 @@ class RelaxServer:
         self.S = n_slots
         self.budget = hbm_budget_bytes
         self.chunk = chunk
+        self.in_flight = max(1, int(in_flight))
         n = 3 * n_max + 3
         self.stride = n
         dev = self.dev
@@ -85,15 +116,24 @@ class RelaxServer:
         self.ntree = torch.zeros(n_slots, dtype=torch.int32, device=dev)
         self.status = torch.zeros(n_slots, dtype=torch.int32, device=dev)
         self.loss = torch.zeros(n_slots, dtype=torch.float32, device=dev)
-        self.order = torch.zeros(n_slots, dtype=torch.int32, device=dev)
+        self.order = torch.zeros((self.in_flight + 1) * n_slots, dtype=torch.int32, device=dev)
         self.stats = torch.zeros(2, dtype=torch.int64, device=dev)
-        self.status_h = torch.zeros(n_slots, dtype=torch.int32).pin_memory()
+        # host staging of admitted poses (slot s at s * stride) and of the per-step lists
+        self.x_host = torch.zeros(n_slots * n, dtype=torch.float32).pin_memory()
+        self.order_h = torch.zeros((self.in_flight + 1) * n_slots, dtype=torch.int32).pin_memory()
+        self.adm_h = torch.zeros(2 * n_slots, dtype=torch.int64).pin_memory()
+        self.status_h = [torch.zeros(n_slots, dtype=torch.int32).pin_memory() for _ in range(self.in_flight + 1)]
         self.stream = torch.cuda.Stream(device=dev)
+        self.side = [torch.cuda.Stream(device=dev) for _ in range(len(R_CLASS_MAX))]
+        self.retire_stream = torch.cuda.Stream(device=dev)
         self._dg = device_geometry(dev, self.geometry)
         self.free = list(range(n_slots))[::-1]
         self.busy: dict[int, Request] = {}
+        self.pending: list[int] = []       # admitted slots whose device reset is not yet enqueued
         self.bytes_in_use = 0
         self.per_slot = per
+        self._steps: deque = deque()
+        self._k = 0                        # steps launched (selects the staging buffers)
 
     # ---------------------------------------------------------------- admission
     def check_request(self, req: Request) -> None:
@@ -101,28 +141,41 @@ class RelaxServer:
         history into the next slot, so it is refused."""
         check_request(req, self.n_max)
 
-    def _admit(self, req: Request, staged: torch.Tensor, now: float) -> None:
+    def _admit(self, req: Request, now: float) -> None:
         self.check_request(req)
         s = self.free.pop()
         n = req.xyt.shape[0]
         o = s * self.stride
+        xh = self.x_host[o:o + 3 * n + 3].numpy()
+        xh[:3 * n] = np.asarray(req.xyt, dtype=np.float32).reshape(-1)
+        xh[3 * n:] = np.asarray(req.h, dtype=np.float32).reshape(-1)
         with torch.cuda.stream(self.stream):
-            self.x[o:o + 3 * n].copy_(staged[:3 * n], non_blocking=True)
-            self.x[o + 3 * n:o + 3 * n + 3].copy_(staged[3 * n:3 * n + 3], non_blocking=True)
-            self.state[s * N.PK_LBFGS_SLOT_BYTES:(s + 1) * N.PK_LBFGS_SLOT_BYTES].zero_()
-            self.status[s:s + 1].zero_()
-            self.ntree[s:s + 1].fill_(n)
+            self.x[o:o + 3 * n + 3].copy_(self.x_host[o:o + 3 * n + 3], non_blocking=True)
         req.slot = s
         req.t_admit = now
         self.busy[s] = req
+        self.pending.append(s)
         self.bytes_in_use += self.per_slot
 
-    def _launch(self) -> None:
-        slots = sorted(self.busy, key=lambda s: -self.busy[s].xyt.shape[0])
-        ordh = torch.tensor(slots, dtype=torch.int32).pin_memory()
-        o = self.opts
+    def _reset_pending(self) -> None:
+        """Fresh optimizer records, status and tree counts for the slots admitted since the
+        last step (one index copy for all of them)."""
+        if not self.pending:
+            return
+        k = len(self.pending)
+        a = self.adm_h.numpy()
+        a[:k] = self.pending
+        a[k:2 * k] = [self.busy[s].xyt.shape[0] for s in self.pending]
+        self.pending = []
         with torch.cuda.stream(self.stream):
-            self.order[:len(slots)].copy_(ordh, non_blocking=True)
+            d = self.adm_h[:2 * k].to(self.dev, non_blocking=True)
+            idx, nt = d[:k], d[k:].to(torch.int32)
+            self.state.view(self.S, -1).index_fill_(0, idx, 0)
+            self.status.index_fill_(0, idx, 0)
+            self.ntree.index_copy_(0, idx, nt)
+
+    def _params(self, order_ptr: int, n_launch: int, n_tree_max: int) -> N.RelaxParams:
+        o = self.opts
         p = N.RelaxParams()
         p.n_slots = self.S
         p.xoff = self.xoff.data_ptr()
@@ -130,9 +183,9 @@ class RelaxServer:
         p.x, p.g = self.x.data_ptr(), self.g.data_ptr()
         p.s_hist, p.y_hist = self.sh.data_ptr(), self.yh.data_ptr()
         p.state = self.state.data_ptr()
-        p.order = self.order.data_ptr()
-        p.n_launch = len(slots)
-        p.n_tree_max = max(self.busy[s].xyt.shape[0] for s in slots)
+        p.order = order_ptr
+        p.n_launch = n_launch
+        p.n_tree_max = n_tree_max
         p.mode = MODES[o.mode]
         p.w_overlap, p.w_boundary, p.w_area = o.w_overlap, o.w_boundary, o.w_area
         p.fixed_h = int(o.fixed_h)
@@ -146,88 +199,136 @@ class RelaxServer:
         p.stop_on_increase = int(o.stop_on_increase)
         p.iters = self.chunk
         from .autotune import choose_block
-        p.block = o.block or choose_block(o.mode, p.n_tree_max, self.geometry.n_pieces)
+        p.block = o.block or choose_block(o.mode, n_tree_max, self.geometry.n_pieces, o.history)
         p.status_out, p.loss_out, p.stats = self.status.data_ptr(), self.loss.data_ptr(), self.stats.data_ptr()
-        N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), self.stream.cuda_stream), "pk_relax_step")
-        with torch.cuda.stream(self.stream):
-            self.status_h.copy_(self.status, non_blocking=True)
-        self._ev = torch.cuda.Event()
-        self._ev.record(self.stream)
-        self._keep = ordh
+        return p
 
-    def _retire(self, now: float, done_list: list) -> None:
-        """Free every slot of the last launch whose status reads finished."""
-        sh = self.status_h.numpy()
-        fin = [s for s in self._launched if sh[s] == 1]
+    def _launch(self) -> None:
+        """One step: every busy slot advanced by chunk iterations, one launch per size
+        class on forked streams, then the status copy this step is retired from."""
+        self._reset_pending()
+        slots = list(self.busy)
+        sizes = np.array([self.busy[s].xyt.shape[0] for s in slots])
+        cls = np.array([size_class(int(n)) for n in sizes])
+        buf = self._k % (self.in_flight + 1)
+        base = buf * self.S
+        oh = self.order_h.numpy()
+        groups, pos = [], base
+        for c in sorted(set(cls.tolist()), reverse=True):
+            sel = np.where(cls == c)[0]
+            sel = sel[np.argsort(-sizes[sel], kind="stable")]
+            oh[pos:pos + len(sel)] = [slots[i] for i in sel]
+            groups.append((pos, len(sel), int(sizes[sel].max())))
+            pos += len(sel)
+        main = self.stream
+        lib = N.lib()
+        with torch.cuda.stream(main):
+            self.order[base:pos].copy_(self.order_h[base:pos], non_blocking=True)
+            fork = torch.cuda.Event()
+            fork.record(main)
+        joins = []
+        for k, (off, cnt, nmax) in enumerate(groups):
+            st = main if k == 0 else self.side[k - 1]
+            if k:
+                st.wait_event(fork)
+            p = self._params(self.order.data_ptr() + 4 * off, cnt, nmax)
+            N.check(lib.pk_relax_step(self._dg.handle, C.byref(p), st.cuda_stream), "pk_relax_step")
+            if k:
+                e = torch.cuda.Event()
+                e.record(st)
+                joins.append(e)
+        for e in joins:
+            main.wait_event(e)
+        sh = self.status_h[buf]
+        with torch.cuda.stream(main):
+            sh.copy_(self.status, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(main)
+        self._steps.append(_Step(ev, sh, {s: self.busy[s] for s in slots}))
+        self._k += 1
+
+    def _retire(self, step: _Step, t0: float, done_list: list) -> None:
+        """Retire the slots that the finished step left done: results and feasibility on the
+        retire stream (the next step keeps running), then back to the host."""
+        st = step.status_h.numpy()
+        fin = [s for s, r in step.slots.items() if st[s] == 1 and self.busy.get(s) is r]
         if not fin:
             return
         reqs = [self.busy.pop(s) for s in fin]
         from .feasibility import overlapping_pairs
-        with torch.cuda.stream(self.stream):
-            res = [self.x[s * self.stride:s * self.stride + 3 * r.xyt.shape[0]].view(-1, 3).clone()
-                   for s, r in zip(fin, reqs)]
-            sizes = [r.xyt.shape[0] for r in reqs]
-            off = torch.tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)).to(self.dev, non_blocking=True)
-            pairs = overlapping_pairs(torch.cat(res, 0), off, max(sizes), stream=self.stream,
-                                      geometry=self.geometry)
+        rs = self.retire_stream
+        rs.wait_event(step.event)
+        sizes = [r.xyt.shape[0] for r in reqs]
+        with torch.cuda.stream(rs):
+            # done slots are never written again by a step, so reading them races with nothing
+            flat = torch.cat([self.x[s * self.stride:s * self.stride + 3 * n] for s, n in zip(fin, sizes)])
+            off = torch.tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)).to(self.dev,
+                                                                                          non_blocking=True)
+            pairs = overlapping_pairs(flat.view(-1, 3), off, max(sizes), stream=rs, geometry=self.geometry)
             idx = torch.tensor(fin, dtype=torch.int64).to(self.dev, non_blocking=True)
             losses = self.loss[idx]
-            self.ntree[idx] = 0
-        self.stream.synchronize()
-        pairs = pairs.cpu().numpy()
-        losses = losses.cpu().numpy()
-        for k, (s, r) in enumerate(zip(fin, reqs)):
+            host = torch.empty(flat.numel(), dtype=torch.float32).pin_memory()
+            host.copy_(flat, non_blocking=True)
+            pairs_h = pairs.to("cpu", non_blocking=True)
+            loss_h = losses.to("cpu", non_blocking=True)
+        rs.synchronize()
+        now = time.perf_counter() - t0
+        k = 0
+        for j, (s, r) in enumerate(zip(fin, reqs)):
+            n = sizes[j]
+            r.result = host[k:k + 3 * n].view(n, 3)
+            k += 3 * n
             r.t_done = now
-            r.result = res[k]
-            r.overlapping_pairs = int(pairs[k])
-            r.loss = float(losses[k])
+            r.overlapping_pairs = int(pairs_h[j])
+            r.loss = float(loss_h[j])
             self.free.append(s)
             self.bytes_in_use -= self.per_slot
         done_list.extend(reqs)
 
     # ---------------------------------------------------------------- session
-    def serve(self, requests: list, realtime: bool = True, poll_s: float = 5e-5) -> dict:
-        """Run a session: requests sorted by arrival (seconds from start).  With realtime
-        the host admits a request only once its arrival time has passed; admissions made
-        while a launch runs are queued on the stream and join the next launch."""
+    def serve(self, requests: list, realtime: bool = True, poll_s: float = 2e-5) -> dict:
+        """Run a session: requests sorted by arrival (seconds from start).  With realtime a
+        request is admitted only once its arrival time has passed (latency = arrival to
+        result in host memory); without, all are available at the start (saturation:
+        throughput = capacity; latency = admission to result)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
             self.check_request(r)
-        # inputs resident in HBM before the session starts
-        staged = [torch.tensor(np.concatenate([r.xyt.reshape(-1), r.h]).astype(np.float32), device=self.dev)
-                  for r in reqs]
         torch.cuda.synchronize()
         done: list = []
         k = 0
-        t0 = time.perf_counter()
-        inflight = False
         n_launch = 0
+        t0 = time.perf_counter()
         while len(done) < len(reqs):
             now = time.perf_counter() - t0
             while k < len(reqs) and self.free and (not realtime or reqs[k].arrival <= now) \
                     and self.bytes_in_use + self.per_slot <= self.budget:
-                self._admit(reqs[k], staged[k], now)
+                self._admit(reqs[k], now)
                 k += 1
-            if not inflight and self.busy:
-                self._launched = list(self.busy)
+            if len(self._steps) < self.in_flight and self.busy:
                 self._launch()
                 n_launch += 1
-                inflight = True
-            if inflight:
-                if self._ev.query():
-                    inflight = False
-                    self._retire(time.perf_counter() - t0, done)
-                else:
-                    time.sleep(poll_s)
+                continue
+            if self._steps and self._steps[0].event.query():
+                self._retire(self._steps.popleft(), t0, done)
+                continue
+            if self._steps:
+                time.sleep(poll_s)
             elif k < len(reqs):
                 time.sleep(max(0.0, min(reqs[k].arrival - (time.perf_counter() - t0), 0.01)))
+        while self._steps:                       # steps launched after the last retirement
+            self._steps.popleft().event.synchronize()
         wall = time.perf_counter() - t0
-        lat = np.array([r.t_done - r.arrival for r in done]) if realtime else \
-            np.array([r.t_done - r.t_admit for r in done])
+        arr = np.array([r.arrival if realtime else r.t_admit for r in done])
+        lat = np.array([r.t_done for r in done]) - arr
+        wait = np.array([r.t_admit for r in done]) - arr
+        svc = np.array([r.t_done - r.t_admit for r in done])
         return {"requests": len(done), "wall_s": wall, "throughput": len(done) / wall,
                 "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                "queue_p50_s": float(np.percentile(wait, 50)), "queue_p99_s": float(np.percentile(wait, 99)),
+                "service_p50_s": float(np.percentile(svc, 50)), "service_p99_s": float(np.percentile(svc, 99)),
                 "launches": n_launch, "feasible_fraction": float(np.mean([r.overlapping_pairs == 0 for r in done])),
-                "slots": self.S, "slot_bytes": self.per_slot}
+                "slots": self.S, "slot_bytes": self.per_slot, "slot_table_bytes": self.S * self.per_slot}
 
 
 def serve_plain(requests: list, opts: RelaxOptions, realtime: bool = True, geometry=None, device="cuda") -> dict:
@@ -245,7 +346,7 @@ def serve_plain(requests: list, opts: RelaxOptions, realtime: bool = True, geome
         r.t_admit = time.perf_counter() - t0
         b.run_plain()
         r.overlapping_pairs = int(overlapping_pairs(b.xyt(0)[None].contiguous(), geometry=geometry)[0])
-        r.result = b.xyt(0).clone()
+        r.result = b.xyt(0).cpu()
         r.t_done = time.perf_counter() - t0
     wall = time.perf_counter() - t0
     lat = np.array([r.t_done - r.arrival for r in reqs])

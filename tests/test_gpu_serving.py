@@ -33,7 +33,7 @@ def test_served_equals_alone(mode):
     for r in reqs[::7]:
         b = RelaxBatch([r.xyt], r.h[None], opts, geometry=geom)
         b.run()
-        assert torch.equal(b.xyt(0), r.result), r.rid
+        assert torch.equal(b.xyt(0).cpu(), r.result), r.rid
 
 
 def test_budget_limits_concurrency():
@@ -57,3 +57,32 @@ def test_budget_limits_concurrency():
     assert rep["requests"] == 12 and max(seen) <= 3
     with pytest.raises(MemoryError):
         RelaxServer(opts, n_slots=4, n_max=40, hbm_budget_bytes=per - 1)
+
+
+def test_mixed_sizes_saturation_and_one_step_in_flight():
+    """Requests of every size class at once (saturation), served with two steps in flight
+    and with one: the same results, each bitwise the result of relaxing it alone, and
+    every request's latency covers its host-to-device copy and its result's return."""
+    from packing_amd.context import set_geometry
+    from packing_amd.pack_cuda_lut import LookupTable
+    from packing_amd.relax import RelaxBatch, RelaxOptions
+    from packing_amd.serving import RelaxServer, poisson_requests, size_class
+    geom = GEOMS["random"]()
+    set_geometry(geom)
+    lut = LookupTable.build_exact(96, 96, 64, full_x=True)
+    opts = RelaxOptions(mode="lut", lut=lut, max_iter=50, history=8, max_step=0.1)
+    reqs = poisson_requests(30, rate=1e6, n_lo=10, n_hi=200, seed_requests=11)
+    assert len({size_class(r.xyt.shape[0]) for r in reqs}) >= 3
+    out = {}
+    for fl in (2, 1):
+        rq = poisson_requests(30, rate=1e6, n_lo=10, n_hi=200, seed_requests=11)
+        srv = RelaxServer(opts, n_slots=12, n_max=200, chunk=20, in_flight=fl)
+        rep = srv.serve(rq, realtime=False)
+        assert rep["requests"] == 30 and rep["service_p50_s"] > 0
+        assert all(r.result.device.type == "cpu" and r.t_done > r.t_admit for r in rq)
+        out[fl] = {r.rid: r.result for r in rq}
+    for r in reqs[::4]:
+        b = RelaxBatch([r.xyt], r.h[None], opts, geometry=geom)
+        b.run()
+        a = b.xyt(0).cpu()
+        assert torch.equal(a, out[2][r.rid]) and torch.equal(a, out[1][r.rid]), r.rid

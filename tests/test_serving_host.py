@@ -33,3 +33,11 @@ def test_oversized_request_is_refused():
         check_request(Request(2, np.zeros((0, 3), np.float32), np.ones(3, np.float32)), 40)
     with pytest.raises(ValueError):
         check_request(Request(3, np.zeros((10, 2), np.float32), np.ones(3, np.float32)), 40)
+
+
+def test_size_classes():
+    """Size classes follow the one-wave kernel's register rows R = ceil((3N + 3) / 64):
+    R <= 3, 4-5, 6-8, 9-10, then the workgroup kernel."""
+    from packing_amd.serving import size_class
+    assert [size_class(n) for n in (1, 63, 64, 105, 106, 169, 170, 212, 213, 400)] == \
+        [0, 0, 1, 1, 2, 2, 3, 3, 4, 4]
