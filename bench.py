@@ -200,8 +200,10 @@ def bucket_roofline(b, sizes, opts, chunk, n_chunks, stream):
         q["block"] = b._block_for(q["n_max"])
         q["tile_source"] = "tiles.json" if choose_block(opts.mode, q["n_max"], b.geometry.n_pieces,
                                                         opts.history) else "library rule"
-        q["kernel"] = f"pk::k_relax_w<2, {-(-(3 * q['n_max'] + 3) // 64)}, 8>" if q["block"] in (0, 1) else \
-            f"pk::k_relax<2, {q['block']}, ...>"
+        rows = -(-(3 * q["n_max"] + 3) // 64)
+        blk = q["block"] or (1 if rows <= 3 else 2)      # 0: the library's rule
+        q["kernel"] = (f"pk::k_relax_w<2, {rows}, 8, 1>" if blk == 1 else
+                       "pk::k_relax_w<2, 10, 8, 2>" if blk == 2 else f"pk::k_relax<2, {blk}, ...>")
     dom = max(per, key=lambda q: sum(q["ms"]))
     avg_s = float(np.mean(dom["ms"])) / 1e3
     alg = dom["bytes"] / n_chunks

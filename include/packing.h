@@ -171,11 +171,14 @@ typedef struct {
     float lr, max_step, tol_grad, tol_change, tol_rel;   /* max_step <= 0: no clamp  */
     int stop_on_increase;
     int iters;                 /* iterations this call                                */
-    int block;                 /* 0: library's choice (1 when N <= 212 and history
-                                  <= 8, else 256); 1: one wave per instance, state
-                                  and history in registers (PK_ERANGE beyond those
-                                  limits); 64, 128, 256: threads per instance.  Every
-                                  choice gives the same bits.                        */
+    int block;                 /* 0: library's choice (N <= 212 and history <= 8:
+                                  1 up to 63 trees, 2 above; else 256); 1: one wave
+                                  per instance, state
+                                  and history in registers; 2: two waves per
+                                  instance, each holding half the registers (1 and 2:
+                                  PK_ERANGE beyond those limits); 64, 128, 256:
+                                  threads per instance.  Every choice gives the same
+                                  bits.                                              */
     int32_t* status_out;       /* [n_slots] or NULL                                   */
     float* loss_out;           /* [n_slots] or NULL                                   */
     unsigned long long* stats; /* [2] tree pairs tested-in, pairs clipped/looked up   */
@@ -187,8 +190,8 @@ typedef struct {
 int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
 
 /* Dynamic LDS bytes per workgroup of pk_relax_step for the given mode, largest tree count,
- * history length and block (1, 64, 128, 256), with the history in LDS (hist_lds 1) or not
- * (0; ignored for block 1, whose history is in registers). */
+ * history length and block (1, 2, 64, 128, 256), with the history in LDS (hist_lds 1) or
+ * not (0; ignored for blocks 1 and 2, whose history is in registers). */
 int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max, int history, int hist_lds,
                        int block);
 

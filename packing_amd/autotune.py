@@ -1,8 +1,8 @@
 """Tile choice for the relaxation kernels, measured per size bucket.
 
 Candidates: 1 = one wave per instance with the state and L-BFGS history in registers
-(up to 212 trees, history <= 8), or 64 / 128 / 256 threads per instance with the vectors
-in LDS.  Every candidate gives bitwise the same results (all reductions are fixed-order
+(up to 212 trees, history <= 8), 2 = the same registers split over two waves (rows 0-4
+and 5-9), or 64 / 128 / 256 threads per instance with the vectors in LDS.  Every candidate gives bitwise the same results (all reductions are fixed-order
 or fixed-point), so the choice only affects speed.
 
 ``tune_batch`` times every candidate on each size bucket of a RelaxBatch (the bucket's
@@ -18,7 +18,7 @@ from __future__ import annotations
 import json
 import os
 
-BLOCKS = (1, 64, 128, 256)
+BLOCKS = (1, 2, 64, 128, 256)
 WAVE_MAX_TREES = 212
 WAVE_MAX_HISTORY = 8
 
@@ -63,7 +63,7 @@ def choose_block(mode: str, n_max: int, n_pieces: int, history: int = 8) -> int:
 
 def candidates(n_max: int, history: int):
     for blk in BLOCKS:
-        if blk == 1 and (n_max > WAVE_MAX_TREES or history > WAVE_MAX_HISTORY):
+        if blk in (1, 2) and (n_max > WAVE_MAX_TREES or history > WAVE_MAX_HISTORY):
             continue
         yield blk
 

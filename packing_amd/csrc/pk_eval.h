@@ -757,9 +757,13 @@ struct Evaluator {
                 }
             }
         } else {
-            for (int base = wave * WAVE; base < N; base += BLOCK) {
+            // each wave walks a contiguous share of the trees (ceil(N / waves)), so that
+            // every wave of a workgroup has candidates even when N < 64 x waves
+            const int share = (N + NW - 1) / NW;
+            const int lo = wave * share, hi = min(N, lo + share);
+            for (int base = lo; base < hi; base += WAVE) {
                 const int i = base + lane;
-                bool alive = i < N;
+                bool alive = i < hi;
                 uint32_t ci = 0, target = 0;
                 T xi = T(0), yi = T(0);
                 int s = 0, j = -1;

@@ -305,10 +305,11 @@ int pk_overlap(pk_geometry_t h, int dtype, const void* xyt, const int32_t* inst_
     PK_REQUIRE(xyt && out_cost, "pk_overlap: xyt and out_cost are required");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (dtype == PK_F32) {
-        // library's choice: one wave per instance up to 150 trees, two above (measured on
-        // MI355X at 50, 100 and 200 trees; 256 threads never won); results are the same
-        // bits for every block
-        if (block == 0) block = n_trees <= 150 ? 64 : 128;
+        // library's choice: four waves per instance, each walking a quarter of the trees
+        // (measured on MI355X at 30, 50, 100, 150 and 200 trees: fastest or tied with 128
+        // everywhere, 1.4x faster than one wave at 100 trees); results are the same bits
+        // for every block
+        if (block == 0) block = 256;
         switch (block) {
             case 64:
                 return dispatch_overlap<float, 64>(g, xyt, inst_off, n_ens, n_trees, mode, crystal,

@@ -12,9 +12,10 @@
 // reference's convergence tests.
 //
 // Two kernels: k_relax, one workgroup per instance with the vectors (and the history
-// when it fits) in LDS and the L-BFGS arithmetic on wave 0; k_relax_w, one wave per
-// instance with the state and history in registers.  Every reduction has a fixed order,
-// so both give the same bits for any workgroup size and may continue each other's runs.
+// when it fits) in LDS and the L-BFGS arithmetic on wave 0; k_relax_w, one or two waves
+// per instance with the state and history in registers.  Every reduction has a fixed
+// order, so all give the same bits for any workgroup size and may continue each other's
+// runs.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -206,6 +207,15 @@ struct Hist {
 // (chronological in k_relax, by ring slot in k_relax_w) does not change them; the
 // direction adds its history terms in ring-slot order in both kernels.
 // ---------------------------------------------------------------------------------
+
+// Canonical order of every sum over a state vector, in every kernel and layout: element k
+// is row k / 64 of lane k % 64; rows 0-4 form group A and rows 5 and above group B; per
+// lane each group's terms are chained in increasing row order; each group is summed over
+// the 64 lanes by one butterfly; the two group totals are added, A + B (B = +0 when the
+// vector has no row beyond 4).  A workgroup of two waves can then hold an instance, one
+// group per wave, and produce the bits of one wave (or one LDS pass) holding both.
+constexpr int ROWS_A = 5;
+constexpr int ELEMS_A = ROWS_A * WAVE;
 
 constexpr int pow2_ceil(int k) { return k <= 1 ? 1 : 2 * pow2_ceil((k + 1) / 2); }
 
@@ -414,17 +424,34 @@ __device__ inline void hist_offsets(const Hist<HL>& H, int head, int m, int (&of
 // Dots of v with the first nv history rows (LDS / global history).  Wave 0 only.
 template <int KH, bool HL, typename VF>
 __device__ inline float lb_dots(int np, int nv, int head, int m, const Hist<HL>& H, VF v) {
-    float acc[Lb<KH>::KD];
-#pragma unroll
-    for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
     int off[KH];
     hist_offsets<KH, HL>(H, head, m, off);
-    for (int k = threadIdx.x; k < np; k += WAVE) {
-        float sv[KH], yv[KH];
-        hist_elem<KH, HL>(H, off, k, nv, sv, yv);
-        Lb<KH>::dots(acc, sv, yv, v(k), nv);
-    }
-    return wave_sum_many_f<Lb<KH>::KD>(acc);
+    auto group = [&](int k0, int k1) {
+        float acc[Lb<KH>::KD];
+#pragma unroll
+        for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
+        for (int k = k0 + (int)threadIdx.x; k < k1; k += WAVE) {
+            float sv[KH], yv[KH];
+            hist_elem<KH, HL>(H, off, k, nv, sv, yv);
+            Lb<KH>::dots(acc, sv, yv, v(k), nv);
+        }
+        return wave_sum_many_f<Lb<KH>::KD>(acc);
+    };
+    const float ra = group(0, np < ELEMS_A ? np : ELEMS_A);
+    const float rb = np > ELEMS_A ? group(ELEMS_A, np) : 0.f;
+    return ra + rb;
+}
+
+// y.s and y.y (or s.y, y.y of a stored pair) summed in the canonical order.  Wave 0 only.
+template <typename PF>
+__device__ inline double lb_pair_sums(int np, PF pf) {
+    float pa[2] = {0.f, 0.f}, pb[2] = {0.f, 0.f};
+    const int ka = np < ELEMS_A ? np : ELEMS_A;
+    for (int k = threadIdx.x; k < ka; k += WAVE) pf(k, pa);
+    for (int k = ELEMS_A + threadIdx.x; k < np; k += WAVE) pf(k, pb);
+    const double ra = wave_sum_many<2>(pa);
+    const double rb = np > ELEMS_A ? wave_sum_many<2>(pb) : 0.0;
+    return ra + rb;
 }
 
 // Rebuild S^T Y and Y^T Y from the stored history rows (the same sums as the incremental
@@ -435,9 +462,7 @@ __device__ inline void lb_rebuild(int np, int hl, int head, int m, const Hist<HL
     for (int j = 0; j < hl; ++j) {
         const int oj = ring(head, j, m) * H.stride;
         const float r = lb_dots<KH, HL>(np, j, head, m, H, [&](int k) { return H.Y(oj, k); });
-        float pa[2] = {0.f, 0.f};
-        for (int k = threadIdx.x; k < np; k += WAVE) Lb<KH>::pair(pa, H.Y(oj, k), H.S(oj, k));
-        const double rp = wave_sum_many<2>(pa);
+        const double rp = lb_pair_sums(np, [&](int k, float (&a)[2]) { Lb<KH>::pair(a, H.Y(oj, k), H.S(oj, k)); });
         Lb<KH>::rebuild_col(r, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), j, head, m, SY, YY);
     }
 }
@@ -459,8 +484,8 @@ __device__ inline void lb_direction(int np, int hl, int head, int m, float gamf,
     int off[KH];
 #pragma unroll
     for (int i = 0; i < KH; ++i) off[i] = (i < m ? i : 0) * H.stride;
-    double sv2[2] = {0.0, 0.0}, mv[2] = {0.0, 0.0};
-    for (int k = threadIdx.x; k < np; k += WAVE) {
+    double sa[2] = {0.0, 0.0}, sb[2] = {0.0, 0.0}, mv[2] = {0.0, 0.0};
+    auto body = [&](int k, double (&sv2)[2]) {
         float sv[KH], yv[KH];
         hist_elem<KH, HL>(H, off, k, hl, sv, yv);
         const float gk = g[k];
@@ -470,8 +495,12 @@ __device__ inline void lb_direction(int np, int hl, int head, int m, float gamf,
         sv2[1] += fabs((double)gk);
         mv[0] = fmax(mv[0], fabs((double)dk));
         mv[1] = fmax(mv[1], fabs((double)x[k]));
-    }
-    const double rs = wave_sum_many<2>(sv2);
+    };
+    const int ka = np < ELEMS_A ? np : ELEMS_A;
+    for (int k = threadIdx.x; k < ka; k += WAVE) body(k, sa);
+    for (int k = ELEMS_A + threadIdx.x; k < np; k += WAVE) body(k, sb);
+    const double rsa = wave_sum_many<2>(sa);
+    const double rs = rsa + (np > ELEMS_A ? wave_sum_many<2>(sb) : 0.0);
     const double rm = wave_max_many<2>(mv);
     gtd = lane_d(rs, many_src<2>(0));
     gsum = lane_d(rs, many_src<2>(1));
@@ -651,16 +680,14 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
             const int niter = S.n_iter;
             const int hl = S.hist_len, head = S.head;
             // history pairs are stored from iteration 2 on (core/lbfgs_torch_parallel.py:636-690)
-            float pa[2] = {0.f, 0.f};
             double mu[2] = {0.0, 0.0};
-            for (int k = lane; k < np; k += WAVE) {
+            const double rp = lb_pair_sums(np, [&](int k, float (&a)[2]) {
                 const float gk = g[k];
                 const float sk = d[k] * tf;
-                Lb<KH>::pair(pa, gk - gp[k], sk);
+                Lb<KH>::pair(a, gk - gp[k], sk);
                 mu[0] = fmax(mu[0], fabs((double)gk));
                 mu[1] = fmax(mu[1], fabs((double)sk));
-            }
-            const double rp = wave_sum_many<2>(pa);
+            });
             const double rmu = wave_max_many<2>(mu);
             const double ys = lane_d(rp, many_src<2>(0)), yy = lane_d(rp, many_src<2>(1));
             const double gm = lane_d(rmu, many_src<2>(0)), pc = lane_d(rmu, many_src<2>(1));
@@ -721,48 +748,118 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     }
 }
 
-// Waves per SIMD the one-wave kernel is compiled for: two for up to 5 rows (the register
-// budget of 256 then holds the state, the history and the evaluator), one above.
-template <int R>
-constexpr int wave_occupancy() { return R <= 5 ? 2 : 1; }
-// Table lookups per lane in flight at once: four for the largest instances (R >= 9, one
-// wave per SIMD whatever the choice), two below, where the registers saved let a wave of
-// a small-instance launch share the SIMD (R <= 8 with R = 3 fits the 512 registers).
-template <int R>
-constexpr int wave_lookups() { return R >= 9 ? 4 : 2; }
+// Waves per SIMD the register kernels are compiled for: two for up to 5 rows per wave (the
+// register budget of 256 then holds the state, the history and the evaluator), one above.
+template <int RW>
+constexpr int wave_occupancy() { return RW <= 5 ? 2 : 1; }
+// Table lookups per lane in flight at once: four for a wave holding 9 or 10 rows (one wave
+// per SIMD whatever the choice), two below, where the registers saved let a second wave
+// share the SIMD.
+template <int RW>
+constexpr int wave_lookups() { return RW >= 9 ? 4 : 2; }
+// Register rows per wave of k_relax_w: all R rows with one wave; with two, wave 0 holds
+// group A (rows 0-4) and wave 1 group B (rows 5-9, zero beyond the state).
+template <int R, int NWV>
+constexpr int rows_per_wave() { return NWV == 1 ? R : ROWS_A; }
+// Occupancy a register kernel is compiled for: that of its rows per wave, except the
+// exact-area evaluator from 4 rows per wave, whose clipping state does not fit the
+// two-wave register budget next to the history (it runs one wave per SIMD, without
+// spills).
+template <int MODE, int R, int NWV>
+constexpr int kernel_occupancy() {
+    return (MODE == MODE_AREA && rows_per_wave<R, NWV>() >= 4) ? 1 : wave_occupancy<rows_per_wave<R, NWV>()>();
+}
 
-// Evaluation of the register state x into g for k_relax_w (through the LDS vectors xs, gs).
-template <int MODE, int R>
+// Totals exchanged between the two waves of a k_relax_w workgroup: each wave writes its
+// group's butterfly totals (one word per value and lane), one barrier, then reads the
+// other wave's.  Two buffers are used in turn, so one barrier per exchange suffices: a
+// buffer is written again only after the barrier of the next exchange, which the other
+// wave reaches only after its reads of this one.
+constexpr int XW = 6;   // words per lane and exchange
+struct Xch {
+    float* buf;    // [2 buffers][2 waves][XW][64]
+    int par, nput, nget;
+    __device__ inline float* at(int wv, int i) const {
+        return buf + ((par * 2 + wv) * XW + i) * WAVE + __lane_id();
+    }
+    __device__ inline void put(float v) { *at(threadIdx.x / WAVE, nput++) = v; }
+    __device__ inline void put(double v) {
+        put(__int_as_float(__double2loint(v)));
+        put(__int_as_float(__double2hiint(v)));
+    }
+    __device__ inline void sync() { __syncthreads(); }
+    __device__ inline float getf() { return *at(1 - (int)(threadIdx.x / WAVE), nget++); }
+    __device__ inline double getd() {
+        const int lo = __float_as_int(getf());
+        const int hi = __float_as_int(getf());
+        return __hiloint2double(hi, lo);
+    }
+    __device__ inline void done() {
+        par ^= 1;
+        nput = nget = 0;
+    }
+};
+
+// Group totals of K float partial lists, a (group A, or this wave's group) and b (group B
+// of a one-wave kernel): one butterfly each, added A + B.  Two waves: a's total is this
+// wave's share; the caller exchanges and adds.
+template <int NWV, int RW, int K>
+__device__ inline float group_sum_f(float (&a)[K], float (&b)[K]) {
+    const float ra = wave_sum_many_f<K>(a);
+    if (NWV == 2) return ra;
+    const float rb = RW > ROWS_A ? wave_sum_many_f<K>(b) : 0.f;
+    return ra + rb;
+}
+template <int NWV, int RW, typename V, int K>
+__device__ inline double group_sum_d(V (&a)[K], V (&b)[K]) {
+    const double ra = wave_sum_many<K>(a);
+    if (NWV == 2) return ra;
+    const double rb = RW > ROWS_A ? wave_sum_many<K>(b) : 0.0;
+    return ra + rb;
+}
+// Row r of this wave belongs to group B (one-wave kernel, r >= 5).
+template <int NWV>
+__device__ constexpr bool in_b(int r) { return NWV == 1 && r >= ROWS_A; }
+
+// Evaluation of the register state x into g for k_relax_w (through the LDS vectors xs, gs;
+// this wave's rows start at row0).
+template <int MODE, int RW, int NWV>
 __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
-                                                const float (&x)[R], float (&g)[R], float* xs, float* gs,
-                                                int N, int n, unsigned long long* bacc, long long& st_tp,
+                                                const float (&x)[RW], float (&g)[RW], float* xs, float* gs,
+                                                int row0, int N, int n, unsigned long long* bacc, long long& st_tp,
                                                 long long& st_pp, int lm, float cut_list, int& nb_valid) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % WAVE;
 #pragma unroll
-    for (int r = 0; r < R; ++r) xs[lane + WAVE * r] = x[r];
+    for (int r = 0; r < RW; ++r) xs[lane + WAVE * (row0 + r)] = x[r];
     __syncthreads();
-    const double loss = relax_eval<MODE, WAVE, wave_lookups<R>()>(G, L, A, xs, gs, N, bacc, st_tp, st_pp, lm,
-                                                                  cut_list);
+    const double loss = relax_eval<MODE, WAVE * NWV, wave_lookups<RW>()>(G, L, A, xs, gs, N, bacc, st_tp,
+                                                                         st_pp, lm, cut_list);
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int k = lane + WAVE * r;
+    for (int r = 0; r < RW; ++r) {
+        const int k = lane + WAVE * (row0 + r);
         g[r] = k < n ? gs[k] : 0.f;
     }
     if (lm == 1) nb_valid = L.wcnt[8] <= L.nb_cap;
     return loss;
 }
 
-// One wave per instance, with the state vectors (x, g, g_prev, d) and the L-BFGS history
-// in registers: element k of a state vector is row k / 64 of lane k % 64 (R rows cover
-// 64 R >= 3N + 3 elements; rows beyond the state are zero), history row i = ring slot i
-// (a new pair overwrites one slot; unused slots are zero).  Only the evaluator's tables
-// and the small matrices live in LDS, so many instances are resident per CU and no
-// workgroup barrier ever waits on idle waves.  The arithmetic is that of k_relax (the
-// same helpers, the same per-lane order), so both kernels give the same bits and may
-// even continue each other's runs.
-template <int MODE, int R, int KH>
-__global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs A) {
+// One or two waves per instance, with the state vectors (x, g, g_prev, d) and the L-BFGS
+// history in registers: element k of a state vector is row k / 64 of lane k % 64 (rows
+// beyond the state are zero); one wave holds all R rows, or two waves hold group A (rows
+// 0-4) and group B (rows 5-9) of an instance of up to 212 trees, halving each wave's
+// registers so that two waves share a SIMD.  History row i = ring slot i (a new pair
+// overwrites one slot; unused slots are zero).  Only the evaluator's tables and the small
+// matrices live in LDS.  Every sum follows the canonical group order, so all variants
+// and k_relax give the same bits and may continue each other's runs.  With two waves
+// both compute the small-matrix solve and every scalar decision from the same exchanged
+// totals; wave 0 alone writes the optimizer record and the matrices.
+template <int MODE, int R, int KH, int NWV>
+__global__ __launch_bounds__(WAVE * NWV, (kernel_occupancy<MODE, R, NWV>())) void k_relax_w(RelaxArgs A) {
+    constexpr int RW = rows_per_wave<R, NWV>();
+    constexpr int RT = RW * NWV;   // rows of the LDS staging vectors
+    static_assert(NWV == 1 || NWV == 2, "one or two waves per instance");
+    static_assert(NWV == 1 || R <= 2 * ROWS_A, "two waves hold at most 10 rows");
     extern __shared__ __align__(16) char smem[];
     __shared__ GeomT<float> G;
     __shared__ LbSlot S;
@@ -774,8 +871,11 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     const int N = p.ntree[slot];
     if (N <= 0) return;
     LbSlot* gS = reinterpret_cast<LbSlot*>(p.state) + slot;
-    const int lane = threadIdx.x;
-    if (lane == 0) S = *gS;
+    const int lane = NWV == 1 ? (int)threadIdx.x : (int)threadIdx.x % WAVE;
+    const int wave = NWV == 1 ? 0 : (int)threadIdx.x / WAVE;
+    const int row0 = wave * RW;
+    const bool lead = threadIdx.x == 0;   // writes the optimizer record
+    if (lead) S = *gS;
     load_geom(G, A.geom);
     __syncthreads();
     if (S.status == ST_DONE) return;
@@ -788,21 +888,24 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     float* __restrict__ Yh = p.y_hist + (long long)m * xoff;
 
     const int Pe = MODE == MODE_LUT ? 0 : G.P;
-    EvalLds<float> L = carve_eval_lds<float>(smem, N, Pe, false, 1, true);
+    EvalLds<float> L = carve_eval_lds<float>(smem, N, Pe, false, NWV, true);
     L.P = G.P;
-    char* vb = smem + eval_lds_bytes<float>(N, Pe, false, 1, true);
-    float* xs = reinterpret_cast<float*>(vb);      // [64 R] evaluator input
-    float* gs = xs + WAVE * R;                     // [64 R] evaluator output
-    double* SY = reinterpret_cast<double*>(gs + WAVE * R);
+    char* vb = smem + eval_lds_bytes<float>(N, Pe, false, NWV, true);
+    float* xs = reinterpret_cast<float*>(vb);      // [64 RT] evaluator input
+    float* gs = xs + WAVE * RT;                    // [64 RT] evaluator output
+    double* SY = reinterpret_cast<double*>(gs + WAVE * RT);
     double* YY = SY + m * m;
+    Xch X2;
+    X2.buf = reinterpret_cast<float*>(YY + m * m);
+    X2.par = X2.nput = X2.nget = 0;
 
-    float x[R], g[R], gp[R], d[R];
-    float hs[KH][R], hy[KH][R];
+    float x[RW], g[RW], gp[RW], d[RW];
+    float hs[KH][RW], hy[KH][RW];
     const bool run0 = S.status == ST_RUN;
     const int hl0 = run0 ? S.hist_len : 0, head0 = S.head;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int k = lane + WAVE * r;
+    for (int r = 0; r < RW; ++r) {
+        const int k = lane + WAVE * (row0 + r);
         const bool in = k < n;
         x[r] = in ? X[k] : 0.f;
         g[r] = in ? Gg[k] : 0.f;
@@ -821,27 +924,42 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
 #pragma unroll
     for (int pc = 0; pc < KH; ++pc) {
         if (pc < hl0) {
-            float acc[Lb<KH>::KD];
+            float aa[Lb<KH>::KD], ab[Lb<KH>::KD];
 #pragma unroll
-            for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
-            float pa[2] = {0.f, 0.f};
+            for (int i = 0; i < Lb<KH>::KD; ++i) aa[i] = ab[i] = 0.f;
+            float pa[2] = {0.f, 0.f}, pb[2] = {0.f, 0.f};
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
+            for (int r = 0; r < RW; ++r) {
                 float sv[KH], yv[KH];
 #pragma unroll
                 for (int i = 0; i < KH; ++i) {
                     sv[i] = hs[i][r];
                     yv[i] = hy[i][r];
                 }
-                Lb<KH>::dots(acc, sv, yv, hy[pc][r], KH);
-                Lb<KH>::pair(pa, hy[pc][r], hs[pc][r]);
+                if (in_b<NWV>(r)) {
+                    Lb<KH>::dots(ab, sv, yv, hy[pc][r], KH);
+                    Lb<KH>::pair(pb, hy[pc][r], hs[pc][r]);
+                } else {
+                    Lb<KH>::dots(aa, sv, yv, hy[pc][r], KH);
+                    Lb<KH>::pair(pa, hy[pc][r], hs[pc][r]);
+                }
             }
-            const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
-            const double rp = wave_sum_many<2>(pa);
-            Lb<KH>::rebuild_col_slot(rr, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), pc, hl0, head0, m,
-                                     SY, YY);
+            float rr = group_sum_f<NWV, RW>(aa, ab);
+            double rp = group_sum_d<NWV, RW>(pa, pb);
+            if (NWV == 2) {
+                X2.put(rr);
+                X2.put(rp);
+                X2.sync();
+                rr = rr + X2.getf();
+                rp = rp + X2.getd();
+                X2.done();
+            }
+            if (wave == 0)
+                Lb<KH>::rebuild_col_slot(rr, lane_d(rp, many_src<2>(0)), lane_d(rp, many_src<2>(1)), pc, hl0,
+                                         head0, m, SY, YY);
         }
     }
+    if (NWV == 2) __syncthreads();   // the matrices wave 0 wrote, before either wave reads them
 
     long long st_tp = 0, st_pp = 0;
     const float skin = 0.5f * G.radius;
@@ -850,13 +968,19 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     double vdisp = 0.0;
 
     if (S.status == ST_FRESH) {
-        const double loss = wave_evaluate<MODE, R>(G, L, A, x, g, xs, gs, N, n, bacc, st_tp, st_pp, 1,
-                                                   cut_list, nb_valid);
+        const double loss = wave_evaluate<MODE, RW, NWV>(G, L, A, x, g, xs, gs, row0, N, n, bacc, st_tp, st_pp,
+                                                         1, cut_list, nb_valid);
         double gm = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) gm = fmax(gm, fabs((double)g[r]));
+        for (int r = 0; r < RW; ++r) gm = fmax(gm, fabs((double)g[r]));
         gm = wave_max(gm);
-        if (lane == 0) {
+        if (NWV == 2) {
+            X2.put(gm);
+            X2.sync();
+            gm = fmax(gm, X2.getd());
+            X2.done();
+        }
+        if (lead) {
             S.loss = loss;
             S.loss0 = loss;
             S.H = 1.0;
@@ -878,26 +1002,35 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     const int li = lane & (KH - 1);
     float a_l = 0.f, b_l = 0.f;
     // rows of this lane that hold a tree centre coordinate (x or y, not theta or h):
-    // k = lane + 64 r with k < 3N and k mod 3 != 2 (64 = 1 mod 3)
+    // k = lane + 64 row with k < 3N and k mod 3 != 2 (64 = 1 mod 3)
     unsigned cmask = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (lane + WAVE * r < 3 * N && (lane + r) % 3 != 2) cmask |= 1u << r;
+    for (int r = 0; r < RW; ++r)
+        if (lane + WAVE * (row0 + r) < 3 * N && (lane + row0 + r) % 3 != 2) cmask |= 1u << r;
     if (S.status == ST_RUN && hl0 > 0) {
-        float acc[Lb<KH>::KD];
+        float aa[Lb<KH>::KD], ab[Lb<KH>::KD];
 #pragma unroll
-        for (int i = 0; i < Lb<KH>::KD; ++i) acc[i] = 0.f;
+        for (int i = 0; i < Lb<KH>::KD; ++i) aa[i] = ab[i] = 0.f;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < RW; ++r) {
             float sv[KH], yv[KH];
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
                 sv[i] = hs[i][r];
                 yv[i] = hy[i][r];
             }
-            Lb<KH>::dots(acc, sv, yv, g[r], KH);
+            if (in_b<NWV>(r))
+                Lb<KH>::dots(ab, sv, yv, g[r], KH);
+            else
+                Lb<KH>::dots(aa, sv, yv, g[r], KH);
         }
-        const float rr = wave_sum_many_f<Lb<KH>::KD>(acc);
+        float rr = group_sum_f<NWV, RW>(aa, ab);
+        if (NWV == 2) {
+            X2.put(rr);
+            X2.sync();
+            rr = rr + X2.getf();
+            X2.done();
+        }
         const int ps = ring(head0, li < m ? li : 0, m);
         a_l = __shfl(rr, many_src<Lb<KH>::KD>(ps));
         b_l = __shfl(rr, many_src<Lb<KH>::KD>(KH + ps));
@@ -906,6 +1039,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     for (int it = 0; it < p.iters; ++it) {
         if (S.status == ST_DONE) break;
         // ---- direction, step length, directional-derivative test, step ----
+        // (every read of the record precedes the exchange below, every write follows it)
         const int niter = S.n_iter + 1;
         const int hl = niter > 1 ? S.hist_len : 0;
         const int head = S.head;
@@ -919,10 +1053,10 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             cs[i] = lane_f(cs_s, i);
             cy[i] = lane_f(cy_s, i);
         }
-        double sv2[2] = {0.0, 0.0};
+        double sa[2] = {0.0, 0.0}, sb[2] = {0.0, 0.0};
         float mv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < RW; ++r) {
             float sv[KH], yv[KH];
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
@@ -932,15 +1066,28 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             const float gk = g[r];
             const float dk = Lb<KH>::dir(gamf, gk, cs, cy, sv, yv);
             d[r] = dk;
-            sv2[0] += (double)gk * (double)dk;
-            sv2[1] += fabs((double)gk);
+            if (in_b<NWV>(r)) {
+                sb[0] += (double)gk * (double)dk;
+                sb[1] += fabs((double)gk);
+            } else {
+                sa[0] += (double)gk * (double)dk;
+                sa[1] += fabs((double)gk);
+            }
             mv[0] = fmaxf(mv[0], fabsf(dk));
             const bool c = (cmask >> r) & 1u;
             mv[1] = fmaxf(mv[1], c ? fabsf(dk) : 0.f);
             mv[2] = fmaxf(mv[2], c ? fabsf(x[r]) : 0.f);
         }
-        const double rs = wave_sum_many<2>(sv2);
-        const float rm = wave_max_many_f<4>(mv);
+        double rs = group_sum_d<NWV, RW>(sa, sb);
+        float rm = wave_max_many_f<4>(mv);
+        if (NWV == 2) {
+            X2.put(rs);
+            X2.put(rm);
+            X2.sync();
+            rs = rs + X2.getd();
+            rm = fmaxf(rm, X2.getf());
+            X2.done();
+        }
         const double gtd = lane_d(rs, many_src<2>(0)), gsum = lane_d(rs, many_src<2>(1));
         const double dmax = (double)lane_f(rm, many_src<4>(0));
         const double dcen = (double)lane_f(rm, many_src<4>(1)), xcen = (double)lane_f(rm, many_src<4>(2));
@@ -949,7 +1096,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         if (p.max_step > 0.f && pc0 > (double)p.max_step) t *= (double)p.max_step / pc0;
         const bool es = gtd > -(double)p.tol_change;
         if (es) {
-            if (lane == 0) {
+            if (lead) {
                 S.status = ST_DONE;
                 S.reason |= R_DIR;
             }
@@ -958,7 +1105,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         }
         const float tf = (float)t;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < RW; ++r) {
             gp[r] = g[r];
             x[r] = x[r] + tf * d[r];
         }
@@ -968,29 +1115,31 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
         vdisp += 1.4142136 * (step * (1.0 + 1e-6) + (xcen + step) * 1.2e-7);
         int lm = (nb_valid && 2.0 * vdisp < 0.9 * (double)skin) ? 2 : 1;
         if (lm == 1) vdisp = 0.0;
-        if (lane == 0) S.n_iter = niter;
+        if (lead) S.n_iter = niter;
         __syncthreads();
         const double loss_prev = S.loss;
         double loss = loss_prev;
         if (niter != p.max_iter)
-            loss = wave_evaluate<MODE, R>(G, L, A, x, g, xs, gs, N, n, bacc, st_tp, st_pp, lm, cut_list,
-                                          nb_valid);
+            loss = wave_evaluate<MODE, RW, NWV>(G, L, A, x, g, xs, gs, row0, N, n, bacc, st_tp, st_pp, lm,
+                                                cut_list, nb_valid);
+        // read before the exchange below, after which wave 0 may update the record
+        const int hl_now = S.hist_len;
         // ---- history update and convergence tests ----
         // One pass over the history rows i in use gives s_i.g and y_i.g for the next
         // direction and s_i.y_new and y_i.y_new for the new S^T Y column and Y^T Y row; the
         // new pair's own products s.g, y.g, y.s, y.y; and max |g|, max |s|.  Each sum has
         // the per-lane partials and the reduction tree of the separate passes of k_relax.
         constexpr int KF = 4 * KH;
-        float acc[KF], an[4] = {0.f, 0.f, 0.f, 0.f}, pa[2] = {0.f, 0.f}, mu[2] = {0.f, 0.f};
+        float fa[KF], fb[KF], na[4] = {0.f, 0.f, 0.f, 0.f}, nb[4] = {0.f, 0.f, 0.f, 0.f};
+        float pa[2] = {0.f, 0.f}, pb[2] = {0.f, 0.f}, mu[2] = {0.f, 0.f};
 #pragma unroll
-        for (int i = 0; i < KF; ++i) acc[i] = 0.f;
-        float yn[R], sn[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int i = 0; i < KF; ++i) fa[i] = fb[i] = 0.f;
+        float yn[RW], sn[RW];
+        auto post_row = [&](int r, float (&acc)[KF], float (&an)[4], float (&pp)[2]) {
             const float gk = g[r];
             yn[r] = gk - gp[r];
             sn[r] = d[r] * tf;
-            Lb<KH>::pair(pa, yn[r], sn[r]);
+            Lb<KH>::pair(pp, yn[r], sn[r]);
             mu[0] = fmaxf(mu[0], fabsf(gk));
             mu[1] = fmaxf(mu[1], fabsf(sn[r]));
 #pragma unroll
@@ -1002,26 +1151,45 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             }
             an[0] = fmaf(sn[r], gk, an[0]);
             an[1] = fmaf(yn[r], gk, an[1]);
+        };
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            if (in_b<NWV>(r))
+                post_row(r, fb, nb, pb);
+            else
+                post_row(r, fa, na, pa);
         }
-        const float rf = wave_sum_many_f<KF>(acc);
-        const float rn = wave_sum_many_f<4>(an);
-        const double rp = wave_sum_many<2>(pa);
-        const float rmu = wave_max_many_f<2>(mu);
+        float rf = group_sum_f<NWV, RW>(fa, fb);
+        float rn = group_sum_f<NWV, RW>(na, nb);
+        double rp = group_sum_d<NWV, RW>(pa, pb);
+        float rmu = wave_max_many_f<2>(mu);
+        if (NWV == 2) {
+            X2.put(rf);
+            X2.put(rn);
+            X2.put(rp);
+            X2.put(rmu);
+            X2.sync();
+            rf = rf + X2.getf();
+            rn = rn + X2.getf();
+            rp = rp + X2.getd();
+            rmu = fmaxf(rmu, X2.getf());
+            X2.done();
+        }
         const double ys = lane_d(rp, many_src<2>(0)), yy = lane_d(rp, many_src<2>(1));
         const double gm = (double)lane_f(rmu, many_src<2>(0)), pc = (double)lane_f(rmu, many_src<2>(1));
-        const int hl_now = S.hist_len;
         const bool acc_ok = niter > 1 && ys > 1e-10;
         const bool full = hl_now >= m;
         const int q = full ? head : hl_now;   // slot of the new pair (the oldest when full)
         if (acc_ok) {
             const int pl = ring(head, li < m ? li : 0, m);
-            Lb<KH>::store_v(__shfl(rf, many_src<KF>(2 * KH + pl)), __shfl(rf, many_src<KF>(3 * KH + pl)), ys, yy,
-                            hl_now, head, m, q, SY, YY, S);
+            if (wave == 0)
+                Lb<KH>::store_v(__shfl(rf, many_src<KF>(2 * KH + pl)), __shfl(rf, many_src<KF>(3 * KH + pl)), ys,
+                                yy, hl_now, head, m, q, SY, YY, S);
 #pragma unroll
             for (int i = 0; i < KH; ++i) {
                 if (i == q) {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
+                    for (int r = 0; r < RW; ++r) {
                         hs[i][r] = sn[r];
                         hy[i][r] = yn[r];
                     }
@@ -1038,7 +1206,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             a_l = is_new ? an_s : ao;
             b_l = is_new ? an_y : bo;
         }
-        if (lane == 0) {
+        if (lead) {
             S.n_eval += 1;
             S.loss = loss;
             int rr = 0;
@@ -1061,8 +1229,8 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     // ---- write back ----
     const int hl_f = S.hist_len;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int k = lane + WAVE * r;
+    for (int r = 0; r < RW; ++r) {
+        const int k = lane + WAVE * (row0 + r);
         if (k < n) {
             X[k] = x[r];
             Gg[k] = g[r];
@@ -1076,7 +1244,7 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
             }
         }
     }
-    if (lane == 0) {
+    if (lead) {
         *gS = S;
         if (p.status_out) p.status_out[slot] = S.status == ST_DONE ? 1 : 0;
         if (p.loss_out) p.loss_out[slot] = (float)S.loss;
@@ -1087,13 +1255,15 @@ __global__ __launch_bounds__(WAVE, wave_occupancy<R>()) void k_relax_w(RelaxArgs
     }
 }
 
-// Dynamic LDS of k_relax_w: evaluator (one wave), evaluator input/output vectors, small
-// matrices.
-size_t relax_w_lds_bytes(int mode, int n_max, int P, int m, int R) {
+// Dynamic LDS of k_relax_w: evaluator (NWV waves), evaluator input/output vectors, small
+// matrices, and (two waves) the exchange buffers.
+size_t relax_w_lds_bytes(int mode, int n_max, int P, int m, int R, int nwv) {
     const int Pe = mode == MODE_LUT ? 0 : P;
-    size_t b = eval_lds_bytes<float>(n_max, Pe, false, 1, true);
-    b += sizeof(float) * 2 * WAVE * (size_t)R;
+    const int rt = nwv == 2 ? 2 * ROWS_A : R;
+    size_t b = eval_lds_bytes<float>(n_max, Pe, false, nwv, true);
+    b += sizeof(float) * 2 * WAVE * (size_t)rt;
     b += sizeof(double) * 2 * (size_t)m * m;
+    if (nwv == 2) b += sizeof(float) * 2 * 2 * XW * WAVE;
     return align_up(b, 16);
 }
 
@@ -1134,39 +1304,41 @@ static int launch_relax(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
     return launch_relax<MODE, BLOCK, false>(g, A, lds_g, st);
 }
 
-// One-wave kernel: rows of the state vector per lane, and the largest supported count.
+// Register kernels: rows of the state vector per lane, and the largest supported count.
 constexpr int W_RMAX = 10;
 constexpr int W_KH = 8;   // history rows held in registers
 inline int wave_rows(int n_tree_max) { return (3 * n_tree_max + 3 + WAVE - 1) / WAVE; }
 
-template <int MODE, int R, int KH>
+template <int MODE, int R, int KH, int NWV>
 static int launch_relax_w_rk(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
     int dev = 0;
     PK_CHECK_HIP(hipGetDevice(&dev));
     int lim = 0;
     PK_CHECK_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t lds = relax_w_lds_bytes(MODE, A.p.n_tree_max, g->hf.P, A.p.history, R);
+    const size_t lds = relax_w_lds_bytes(MODE, A.p.n_tree_max, g->hf.P, A.p.history, R, NWV);
     const size_t stat = sizeof(GeomT<float>) + sizeof(LbSlot) + 64;
     if (lds + stat > (size_t)lim) {
         set_error("pk_relax_step: instance of " + std::to_string(A.p.n_tree_max) + " trees needs " +
                   std::to_string(lds + stat) + " B of LDS, device allows " + std::to_string(lim));
         return PK_ERANGE;
     }
-    auto kern = k_relax_w<MODE, R, KH>;
+    auto kern = k_relax_w<MODE, R, KH, NWV>;
     PK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3(A.p.n_launch), dim3(WAVE), lds, st, A);
+    hipLaunchKernelGGL(kern, dim3(A.p.n_launch), dim3(WAVE * NWV), lds, st, A);
     PK_CHECK_HIP(hipGetLastError());
     return 0;
 }
 
 template <int MODE, int R>
 static int launch_relax_w_r(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
-    return launch_relax_w_rk<MODE, R, W_KH>(g, A, st);
+    return launch_relax_w_rk<MODE, R, W_KH, 1>(g, A, st);
 }
 
+// nwv 1: one wave holding R rows; 2: two waves holding 5 rows each (any R <= 10).
 template <int MODE>
-static int launch_relax_w(const Geometry* g, const RelaxArgs& A, int rows, hipStream_t st) {
+static int launch_relax_w(const Geometry* g, const RelaxArgs& A, int rows, int nwv, hipStream_t st) {
+    if (nwv == 2) return launch_relax_w_rk<MODE, 2 * ROWS_A, W_KH, 2>(g, A, st);
     switch (rows) {
         case 1: return launch_relax_w_r<MODE, 1>(g, A, st);
         case 2: return launch_relax_w_r<MODE, 2>(g, A, st);
@@ -1189,24 +1361,27 @@ static int relax_dispatch(const Geometry* g, const RelaxArgs& A, int block, hipS
     const int rows = wave_rows(p->n_tree_max);
     int blk = block;
     const bool wave_ok = rows <= W_RMAX && p->history <= W_KH;
-    if (blk == 0) blk = wave_ok ? 1 : 256;
-    if (blk == 1) {
+    // library's choice: registers where they fit, one wave up to 3 rows and two above
+    // (measured on MI355X in table mode: two waves 1.15x faster at 5 rows, 1.37x at 10;
+    // one wave faster at 3)
+    if (blk == 0) blk = wave_ok ? (rows <= 3 ? 1 : 2) : 256;
+    if (blk == 1 || blk == 2) {
         if (!wave_ok) {
-            set_error("pk_relax_step: block 1 (one wave per instance) supports up to " +
-                      std::to_string((W_RMAX * WAVE - 3) / 3) + " trees and history " +
+            set_error("pk_relax_step: blocks 1 and 2 (register state, one or two waves per instance) "
+                      "support up to " + std::to_string((W_RMAX * WAVE - 3) / 3) + " trees and history " +
                       std::to_string(W_KH));
             return PK_ERANGE;
         }
-        if (p->mode == PK_MODE_AREA) return launch_relax_w<MODE_AREA>(g, A, rows, st);
-        if (p->mode == PK_MODE_SEP) return launch_relax_w<MODE_SEP>(g, A, rows, st);
-        return launch_relax_w<MODE_LUT>(g, A, rows, st);
+        if (p->mode == PK_MODE_AREA) return launch_relax_w<MODE_AREA>(g, A, rows, blk, st);
+        if (p->mode == PK_MODE_SEP) return launch_relax_w<MODE_SEP>(g, A, rows, blk, st);
+        return launch_relax_w<MODE_LUT>(g, A, rows, blk, st);
     }
 #define PK_RL(M)                                                          \
     switch (blk) {                                                        \
         case 64: return launch_relax<M, 64>(g, A, st);                    \
         case 128: return launch_relax<M, 128>(g, A, st);                  \
         case 256: return launch_relax<M, 256>(g, A, st);                  \
-        default: set_error("pk_relax_step: block must be 0, 1, 64, 128 or 256"); \
+        default: set_error("pk_relax_step: block must be 0, 1, 2, 64, 128 or 256"); \
             return PK_EINVAL;                                             \
     }
     if (p->mode == PK_MODE_AREA) { PK_RL(MODE_AREA) }
@@ -1274,7 +1449,8 @@ int pk_relax_slot_bytes(void) { return (int)sizeof(LbSlot); }
 int pk_relax_lds_bytes(pk_geometry_t h, int mode, int n_tree_max, int history, int hist_lds, int block) {
     const Geometry* g = reinterpret_cast<const Geometry*>(h);
     if (!g || n_tree_max < 1 || history < 1) return PK_EINVAL;
-    if (block == 1) return (int)relax_w_lds_bytes(mode, n_tree_max, g->hf.P, history, wave_rows(n_tree_max));
+    if (block == 1 || block == 2)
+        return (int)relax_w_lds_bytes(mode, n_tree_max, g->hf.P, history, wave_rows(n_tree_max), block);
     if (block != 64 && block != 128 && block != 256) return PK_EINVAL;
     return (int)relax_lds_bytes(mode, n_tree_max, g->hf.P, history, hist_lds != 0, block);
 }

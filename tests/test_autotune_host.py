@@ -9,7 +9,7 @@ def test_size_class_keys():
     assert A.size_class(10) == "R1" and A.size_class(200) == "R10" and A.size_class(212) == "R10"
     assert A.size_class(213) == "N256" and A.size_class(300) == "N512"
     assert A.key("lut", 200, 10, 8) == "lut:R10:P10:m8"
-    assert list(A.candidates(200, 8)) == [1, 64, 128, 256]
+    assert list(A.candidates(200, 8)) == [1, 2, 64, 128, 256]
     assert list(A.candidates(213, 8)) == [64, 128, 256]
     assert list(A.candidates(100, 12)) == [64, 128, 256]
 

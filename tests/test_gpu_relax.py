@@ -84,7 +84,7 @@ def test_invariances():
     ref = batch(geom, xs, hs, **kw)
     ref.run()
     R = [ref.xyt(i).cpu().numpy() for i in range(len(sizes))]
-    for blk in (1, 64, 128, 256):
+    for blk in (1, 2, 64, 128, 256):
         b = batch(geom, xs, hs, block=blk, **kw)
         b.run(iters_per_launch=7)
         for i in range(len(sizes)):
@@ -148,11 +148,13 @@ def test_lut_mode_relaxes():
     assert (b.loss.cpu().numpy() < l0).all()
 
 
-@pytest.mark.parametrize("mode", ["lut", "area"])
+@pytest.mark.parametrize("mode", ["lut", "area", "sep"])
 def test_wave_kernel_matches_lds_kernel(mode):
-    """The one-wave kernel (state and history in registers) and the LDS kernel give the
-    same bits, also when a run switches kernels between launches (the stored history is
-    read back by either), near the one-wave size limit and with a short history."""
+    """The register kernels (one wave holding all rows, two waves holding rows 0-4 and
+    5-9) and the LDS kernel give the same bits, also when a run switches kernels between
+    launches (the stored history is read back by any of them), near the register size
+    limit (212 trees, 10 rows), across the row-5 group boundary and with a short
+    history."""
     from packing_amd.context import set_geometry
     from packing_amd.pack_cuda_lut import LookupTable
     geom = GEOMS["random"]()
@@ -160,7 +162,7 @@ def test_wave_kernel_matches_lds_kernel(mode):
     kw = dict(mode=mode, max_iter=40, history=8, max_step=0.1)
     if mode == "lut":
         kw["lut"] = LookupTable.build_exact(96, 96, 64, full_x=True)
-    sizes = [212, 150, 21, 22, 43, 5] if mode == "lut" else [60, 21, 22, 43, 5]
+    sizes = [212, 150, 21, 22, 43, 5] if mode == "lut" else [120, 106, 60, 21, 5]
     xs, hs = ragged(7, sizes)
     for hist in (8, 3):
         kw["history"] = hist
@@ -168,15 +170,19 @@ def test_wave_kernel_matches_lds_kernel(mode):
         ref.run()
         one = batch(geom, xs, hs, block=1, **kw)
         one.run(iters_per_launch=13)
+        two = batch(geom, xs, hs, block=2, **kw)
+        two.run(iters_per_launch=11)
         mix = batch(geom, xs, hs, block=1, **kw)
-        for k, blk in enumerate((1, 256, 1, 64)):
+        for k, blk in enumerate((1, 2, 256, 1, 2, 64, 2, 1)):
             mix.opts.block = blk
-            mix.launch(10)
+            mix.launch(5)
         for i in range(len(sizes)):
             r = ref.xyt(i).cpu().numpy()
             assert np.array_equal(one.xyt(i).cpu().numpy(), r), ("one", hist, i)
+            assert np.array_equal(two.xyt(i).cpu().numpy(), r), ("two", hist, i)
             assert np.array_equal(mix.xyt(i).cpu().numpy(), r), ("mix", hist, i)
         assert np.array_equal(one.slot_records(), ref.slot_records())
+        assert np.array_equal(two.slot_records(), ref.slot_records())
 
 
 def test_wave_kernel_limits():
@@ -195,9 +201,13 @@ def test_wave_kernel_limits():
     h = batch(geom, *ragged(3, [30]), mode="area", max_iter=3, history=9, max_step=0.1, block=1)
     with pytest.raises(ValueError):
         h.run()
+    for blk in (1, 2):
+        b = batch(geom, xs, hs, mode="area", max_iter=3, history=8, max_step=0.1, block=blk)
+        with pytest.raises(ValueError):
+            b.run()
 
 
-def _iterates_vs_oracle(oracle_mod, geom, xs, hs, tol, opts_kw, oracle_kw, blocks=(1, 128)):
+def _iterates_vs_oracle(oracle_mod, geom, xs, hs, tol, opts_kw, oracle_kw, blocks=(1, 2, 128)):
     """Relax on the GPU with each block choice and compare poses (and h) with the oracle."""
     og = oracle_mod.OracleGeometry(geom)
     worst = 0.0
