@@ -92,12 +92,12 @@ int pk_boundary(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst
  * with an explicit handle.  mirror = 1 reproduces the reference's half table (x >= 0
  * stored, x < 0 looked up as (-x, -theta); core/pack_cuda_lut.py:128-131,314-320);
  * mirror = 0 is a full-range table for polygons without mirror symmetry.
- * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).  At most 2^32 - 1
- * nodes (the reference's default 900^3 fits); a mirrored table needs t_min = -t_max.
- * The handle keeps its own corner-quad copy of vals (4x the table's memory, on the
- * table's device): entry (ix, iy, it) holds the four (x, y) corners of a cell at theta
- * node it, so a lookup reads two adjacent 16-byte entries instead of four scattered
- * pairs.  The copy is taken here (the call synchronizes that device), so writes to vals
+ * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).  Limits: nx ny and
+ * nt + 1 below 2^24, nx ny (nt + 1) below 2^32 (the reference's default 900^3 fits); a
+ * mirrored table needs t_min = -t_max.  The handle keeps its own corner-quad copy of
+ * vals (4 (nt + 1) / nt times the table's memory, on the table's device): entry
+ * (ix, iy, it) holds the four (x, y) corners of a cell at theta node min(it, nt - 1), so
+ * a lookup reads two adjacent 16-byte entries instead of four scattered pairs.  The copy is taken here (the call synchronizes that device), so writes to vals
  * after pk_lut_create are not seen; create a new handle instead.
  */
 int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, double x_max,

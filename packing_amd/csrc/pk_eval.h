@@ -30,8 +30,9 @@ enum { MODE_AREA = 0, MODE_SEP = 1, MODE_LUT = 2 };
 // Pose-space lookup table in device memory: vals[ix][iy][it], theta fastest.
 struct LutDev {
     const float* vals;     // vals[ix][iy][it], theta fastest (the caller's table)
-    const float4* corners; // corners[ix][iy][it] = values at (ix, iy), (ix, iy+1), (ix+1, iy),
-                           // (ix+1, iy+1) and node it, upper indices clamped (library's copy)
+    const float4* corners; // corners[ix][iy][it], it <= nt: values at (ix, iy), (ix, iy+1),
+                           // (ix+1, iy), (ix+1, iy+1) and theta node min(it, nt - 1), upper
+                           // indices clamped (library's copy, nt + 1 entries per (ix, iy))
     int nx, ny, nt;
     float x_lo, x_min, x_max, y_min, y_max, t_min, t_max;   // x_lo = range check low end
     float sx, sy, st;                                       // (n - 1) / (max - min)
@@ -42,13 +43,14 @@ struct LutDev {
 // One table lookup split in two so that the loads of several lookups can be in flight
 // together: lut_fetch computes the cell and issues the loads, lut_combine interpolates.
 // The eight corners of a cell are the four (x, y) corners of the corner copy at theta nodes
-// bt and bt + 1: two adjacent 16-byte entries, one cache line in most cases (the caller's
-// table would need four lines).  top selects node bt + 1 for both theta corners; the
-// selection is left to lut_combine so that nothing waits on the loads before it.
+// it and it + 1: two adjacent 16-byte entries, one cache line in most cases (the caller's
+// table would need four lines).  The copy repeats the last theta node once more, so a
+// lookup at the upper theta end (it = nt - 1) reads node nt - 1 for both theta corners,
+// the reference's clamped upper index, with no selection.
 struct LutTap {
     float4 e0, e1;
     float fx, fy, ft;
-    bool flip, in, top;
+    bool flip, in;
 };
 
 __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, LutTap& t) {
@@ -62,14 +64,13 @@ __device__ inline void lut_fetch(const LutDev& T_, float x, float y, float th, L
     t.fx = gx - ix0;
     t.fy = gy - iy0;
     t.ft = gt - it0;
-    // theta pair (it0, min(it0 + 1, nt - 1)) read as nodes (bt, bt + 1), bt <= nt - 2
-    const int bt = min(it0, T_.nt - 2);
-    t.top = it0 != bt;   // it0 = nt - 1: both corners are node nt - 1
     (void)ix1;
     (void)iy1;   // the corner copy holds the clamped upper corners
-    // 32-bit entry index (pk_lut_create keeps tables below 2^32 nodes), scaled to a 64-bit
-    // address by one shift-add: tables of any size up to that, e.g. 900^3 nodes
-    const uint32_t idx = ((uint32_t)ix0 * (uint32_t)T_.ny + (uint32_t)iy0) * (uint32_t)T_.nt + (uint32_t)bt;
+    // 32-bit entry index from 24-bit products (pk_lut_create keeps nx ny and nt + 1 below
+    // 2^24 and the copy below 2^32 entries, e.g. 900^3 nodes), scaled to a 64-bit address
+    // by one shift-add
+    const uint32_t idx = __umul24(__umul24((uint32_t)ix0, (uint32_t)T_.ny) + (uint32_t)iy0,
+                                  (uint32_t)T_.nt + 1u) + (uint32_t)it0;
     const float4* __restrict__ p = T_.corners + idx;
     t.e0 = p[0];
     t.e1 = p[1];
@@ -81,10 +82,10 @@ __device__ inline float lut_combine(const LutDev& T_, const LutTap& t, float (&g
     g[0] = g[1] = g[2] = 0.f;
     if (!t.in) return 0.f;
     const float fx = t.fx, fy = t.fy, ft = t.ft;
-    const float v000 = t.top ? t.e1.x : t.e0.x, v001 = t.e1.x;
-    const float v010 = t.top ? t.e1.y : t.e0.y, v011 = t.e1.y;
-    const float v100 = t.top ? t.e1.z : t.e0.z, v101 = t.e1.z;
-    const float v110 = t.top ? t.e1.w : t.e0.w, v111 = t.e1.w;
+    const float v000 = t.e0.x, v001 = t.e1.x;
+    const float v010 = t.e0.y, v011 = t.e1.y;
+    const float v100 = t.e0.z, v101 = t.e1.z;
+    const float v110 = t.e0.w, v111 = t.e1.w;
     const float v00 = v000 * (1.f - ft) + v001 * ft, v01 = v010 * (1.f - ft) + v011 * ft;
     const float v10 = v100 * (1.f - ft) + v101 * ft, v11 = v110 * (1.f - ft) + v111 * ft;
     const float v0 = v00 * (1.f - fy) + v01 * fy, v1 = v10 * (1.f - fy) + v11 * fy;

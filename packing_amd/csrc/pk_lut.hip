@@ -15,16 +15,18 @@ struct Lut {
     int device = 0;
 };
 
-// Corner copy of a table: entry (ix, iy, it) holds the values at (ix, iy), (ix, iy+1),
-// (ix+1, iy), (ix+1, iy+1) and theta node it, upper indices clamped to the last node, so
-// the eight corners of a cell are two adjacent entries.
+// Corner copy of a table: entry (ix, iy, it), it = 0..nt, holds the values at (ix, iy),
+// (ix, iy+1), (ix+1, iy), (ix+1, iy+1) and theta node min(it, nt - 1), upper indices
+// clamped to the last node, so the eight corners of a cell are two adjacent entries, also
+// at the upper theta end.
 __global__ __launch_bounds__(256) void k_lut_corners(const float* __restrict__ vals, int nx, int ny, int nt,
                                                      float4* __restrict__ quad) {
-    const long long total = (long long)nx * ny * nt;
+    const int ntc = nt + 1;
+    const long long total = (long long)nx * ny * ntc;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
-        const int it = (int)(q % nt);
-        const long long r = q / nt;
+        const int it = min((int)(q % ntc), nt - 1);
+        const long long r = q / ntc;
         const int iy = (int)(r % ny), ix = (int)(r / ny);
         const int ix1 = min(ix + 1, nx - 1), iy1 = min(iy + 1, ny - 1);
         float4 e;
@@ -141,8 +143,10 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
     PK_REQUIRE(out != nullptr && vals != nullptr, "pk_lut_create: NULL argument");
     PK_REQUIRE(nx >= 2 && ny >= 2 && nt >= 2, "pk_lut_create: every axis needs >= 2 nodes");
     PK_REQUIRE(x_max > x_min && y_max > y_min && t_max > t_min, "pk_lut_create: empty range");
-    const long long nodes = (long long)nx * ny * nt;
-    PK_REQUIRE(nodes < (1ll << 32), "pk_lut_create: at most 2^32 - 1 nodes");
+    const long long nodes = (long long)nx * ny * (nt + 1);   // corner-copy entries
+    PK_REQUIRE(nodes < (1ll << 32), "pk_lut_create: at most 2^32 - 1 entries nx ny (nt + 1)");
+    PK_REQUIRE((long long)nx * ny < (1ll << 24) && nt + 1 < (1 << 24),
+               "pk_lut_create: nx ny and nt + 1 must stay below 2^24");
     // a mirrored lookup indexes (-x, -theta): the theta range must be symmetric about 0, or a
     // relative angle inside [t_min, t_max] could land below the first theta node
     PK_REQUIRE(!mirror || (float)t_min == -(float)t_max,

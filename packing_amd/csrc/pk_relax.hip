@@ -49,11 +49,20 @@ struct RelaxArgs {
 
 __host__ __device__ inline int pad64(int n) { return (n + WAVE - 1) / WAVE * WAVE; }
 
+// Component j of the h gradient after an evaluation (relax_eval's formula), from the
+// boundary sums; x0 = h[0].
+__device__ inline float h_gradient(const pk_relax_params& p, const unsigned long long* bacc, int j, float x0) {
+    if (p.fixed_h) return 0.f;
+    if (j == 0) return p.w_boundary * Acc<float>::get(bacc[1]) + 2.f * p.w_area * x0;
+    return p.w_boundary * Acc<float>::get(bacc[1 + j]);
+}
+
 // Cost and gradient of the state x into g (all threads); returns the loss.  list_mode:
 // 1 rebuilds the neighbour list while evaluating, 2 evaluates from it (no cell grid).
 // Each tree's poses, sincos and boundary term are handled by the same thread, so the
-// staging needs no barrier between them.
-template <int MODE, int BLOCK, int LU = 2>
+// staging needs no barrier between them.  HW: thread 0 writes the h gradient g[3N..3N+2]
+// (visible to its own wave on return); without, the caller forms it with h_gradient.
+template <int MODE, int BLOCK, int LU = 2, bool HW = true>
 __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
                                     const float* x, float* g, int N,
                                     unsigned long long* bacc /* [4] boundary accumulators */,
@@ -158,14 +167,9 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     const double c_ov = Acc<float>::get(*L.cost), c_bd = Acc<float>::get(bacc[0]);
     double loss = (double)p.w_overlap * c_ov + (double)p.w_boundary * c_bd;
     // the h gradient is read by wave 0, which thread 0 belongs to, after this returns
-    if (threadIdx.x == 0) {
-        if (p.fixed_h) {
-            g[3 * N] = g[3 * N + 1] = g[3 * N + 2] = 0.f;
-        } else {
-            g[3 * N] = p.w_boundary * Acc<float>::get(bacc[1]) + 2.f * p.w_area * x[3 * N];
-            g[3 * N + 1] = p.w_boundary * Acc<float>::get(bacc[2]);
-            g[3 * N + 2] = p.w_boundary * Acc<float>::get(bacc[3]);
-        }
+    if (HW && threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g[3 * N + j] = h_gradient(p, bacc, j, x[3 * N]);
     }
     if (!p.fixed_h) loss += (double)p.w_area * (double)x[3 * N] * (double)x[3 * N];
     return loss;
@@ -832,13 +836,15 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
 #pragma unroll
     for (int r = 0; r < RW; ++r) xs[lane + WAVE * (row0 + r)] = x[r];
     __syncthreads();
-    const double loss = relax_eval<MODE, WAVE * NWV, wave_lookups<RW>()>(G, L, A, xs, gs, N, bacc, st_tp,
-                                                                         st_pp, lm, cut_list);
-    __syncthreads();
+    const double loss = relax_eval<MODE, WAVE * NWV, wave_lookups<RW>(), false>(G, L, A, xs, gs, N, bacc,
+                                                                                st_tp, st_pp, lm, cut_list);
+    // relax_eval ends with a barrier after the last write of gs and of the boundary sums;
+    // the lane holding element 3N + j forms the h gradient itself
+    const int n3 = n - 3;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
         const int k = lane + WAVE * (row0 + r);
-        g[r] = k < n ? gs[k] : 0.f;
+        g[r] = k < n3 ? gs[k] : (k < n ? h_gradient(A.p, bacc, k - n3, x[r]) : 0.f);
     }
     if (lm == 1) nb_valid = L.wcnt[8] <= L.nb_cap;
     return loss;
@@ -1115,8 +1121,8 @@ __global__ __launch_bounds__(WAVE * NWV, (kernel_occupancy<MODE, R, NWV>())) voi
         vdisp += 1.4142136 * (step * (1.0 + 1e-6) + (xcen + step) * 1.2e-7);
         int lm = (nb_valid && 2.0 * vdisp < 0.9 * (double)skin) ? 2 : 1;
         if (lm == 1) vdisp = 0.0;
+        // (S.n_iter is read again only at the next iteration, after its barriers)
         if (lead) S.n_iter = niter;
-        __syncthreads();
         const double loss_prev = S.loss;
         double loss = loss_prev;
         if (niter != p.max_iter)

@@ -28,6 +28,11 @@ from .context import device_geometry, get_geometry
 MODES = {"area": N.PK_MODE_AREA, "sep": N.PK_MODE_SEP, "lut": N.PK_MODE_LUT}
 
 
+# Size buckets launched concurrently, one stream each: at most the 4 hardware queues a
+# process gets (more were measured slower and are not exercised).
+MAX_BUCKETS = 4
+
+
 @dataclass
 class RelaxOptions:
     mode: str = "area"
@@ -45,8 +50,14 @@ class RelaxOptions:
     tol_rel: float = 0.0
     stop_on_increase: bool = False
     block: int = 0          # 0: tile table (autotune.py), else the library's choice
-    buckets: int = 4        # size buckets launched concurrently (1: one launch)
+    buckets: int = 4        # size buckets launched concurrently (1: one launch; at most 4,
+                            # the hardware queues of one process)
     hist_lds: int = -1      # L-BFGS history in LDS: -1 when it fits, 0 never, 1 always
+
+    def __post_init__(self):
+        if not 1 <= int(self.buckets or 1) <= MAX_BUCKETS:
+            raise ValueError(f"buckets must be 1..{MAX_BUCKETS} (one stream per hardware queue), "
+                             f"not {self.buckets}")
     lut: object = None      # pack_cuda_lut.LookupTable for mode "lut"
 
 
@@ -109,6 +120,9 @@ class RelaxBatch:
         """Split the size-sorted order into up to opts.buckets equal-count slices:
         (offset, count, n_tree_max).  Dynamic LDS is sized per launch by its largest
         instance, so a slice of small instances gets small workgroups and high occupancy."""
+        if not 1 <= int(self.opts.buckets or 1) <= MAX_BUCKETS:
+            raise ValueError(f"buckets must be 1..{MAX_BUCKETS} (one stream per hardware queue), "
+                             f"not {self.opts.buckets}")
         k = max(1, min(int(self.opts.buckets or 1), self.M))
         if self.M == 0:
             return []

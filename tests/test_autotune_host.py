@@ -24,3 +24,13 @@ def test_choose_block_reads_table(tmp_path, monkeypatch):
     assert A.choose_block("area", 30, 4, 8) == 64
     assert A.choose_block("lut", 50, 10, 8) == 0          # no entry: the library's rule
     monkeypatch.setattr(A, "_table", None)
+
+
+def test_bucket_count_is_limited():
+    """At most four concurrent size buckets (one stream per hardware queue)."""
+    import pytest
+    from packing_amd.relax import RelaxOptions
+    RelaxOptions(buckets=4)
+    RelaxOptions(buckets=1)
+    with pytest.raises(ValueError):
+        RelaxOptions(buckets=6)
