@@ -576,15 +576,30 @@ struct Evaluator {
         float cr, sr, dxl, dyl;
         LutTap tap;
     };
-    __device__ inline void lut_begin(int i, int j, LutPair& q) const {
+    // Poses of a pair as read from LDS (loaded for every lookup of a drain before any is
+    // used, so that the LDS round trips overlap).
+    struct LutPose {
+        float xi, yi, ti, ci, si, xj, yj, tj;
+    };
+    __device__ inline void lut_load(int i, int j, LutPose& o) const {
+        o.xi = L.x[i];
+        o.yi = L.y[i];
+        o.ti = L.t[i];
+        o.ci = L.c[i];
+        o.si = L.s[i];
+        o.xj = L.x[j];
+        o.yj = L.y[j];
+        o.tj = L.t[j];
+    }
+    __device__ inline void lut_begin(int i, int j, const LutPose& o, LutPair& q) const {
         q.i = i;
         q.j = j;
-        const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-        q.cr = L.c[i];
-        q.sr = L.s[i];
+        const T dx = o.xj - o.xi, dy = o.yj - o.yi;
+        q.cr = o.ci;
+        q.sr = o.si;
         q.dxl = q.cr * dx + q.sr * dy;
         q.dyl = -q.sr * dx + q.cr * dy;
-        const float dth = wrap_pi(L.t[j] - L.t[i]);
+        const float dth = wrap_pi(o.tj - o.ti);
         lut_fetch(lut, q.dxl, q.dyl, dth, q.tap);
     }
     __device__ inline void lut_end(const LutPair& q) {
@@ -614,14 +629,18 @@ struct Evaluator {
     // them in flight before any is interpolated.
     __device__ inline void lut_pairs(const uint32_t* q, int cnt) {
         const int lane = threadIdx.x % WAVE;
+        // queue entries, then the poses of every pair (a lane without an entry reads tree
+        // 0's, never used), then the table gathers, then the interpolations
+        uint32_t e[LU];
+#pragma unroll
+        for (int u = 0; u < LU; ++u) e[u] = lane + WAVE * u < cnt ? q[lane + WAVE * u] : 0u;
+        LutPose o[LU];
+#pragma unroll
+        for (int u = 0; u < LU; ++u) lut_load(e[u] & 0xfff, (e[u] >> 12) & 0xfff, o[u]);
         LutPair pr[LU];
 #pragma unroll
-        for (int u = 0; u < LU; ++u) {
-            if (lane + WAVE * u < cnt) {
-                const uint32_t e = q[lane + WAVE * u];
-                lut_begin(e & 0xfff, (e >> 12) & 0xfff, pr[u]);
-            }
-        }
+        for (int u = 0; u < LU; ++u)
+            if (lane + WAVE * u < cnt) lut_begin(e[u] & 0xfff, (e[u] >> 12) & 0xfff, o[u], pr[u]);
 #pragma unroll
         for (int u = 0; u < LU; ++u)
             if (lane + WAVE * u < cnt) lut_end(pr[u]);
