@@ -98,8 +98,10 @@ def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0, n_sat=40
                       geometry=geom)
     warm = poisson_requests(64, rate=rate, seed_arrivals=1, seed_requests=2)
     srv.serve(warm, realtime=False)
+    log("serving: warm-up session done")
     reqs = poisson_requests(n_req, rate=rate)
     rep = srv.serve(reqs)
+    log(f"serving: Poisson session done ({rep['requests']} requests, {rep['wall_s']:.2f} s)")
     ms = lambda v: round(v * 1e3, 3)
     out = {"requests": rep["requests"], "rate_per_s": rate, "iters": iters,
            "throughput": round(rep["throughput"], 2), "p50_ms": ms(rep["p50_s"]), "p99_ms": ms(rep["p99_s"]),
@@ -113,6 +115,7 @@ def serving_section(geom, lut, dev, n_req, n_plain, iters, rate=2000.0, n_sat=40
     # saturation: every request available at the start, so throughput is the server's capacity
     sat = poisson_requests(n_sat, rate=rate, seed_arrivals=8, seed_requests=9)
     rs = srv.serve(sat, realtime=False)
+    log(f"serving: saturation session done ({rs['requests']} requests, {rs['wall_s']:.2f} s)")
     out["saturation"] = {"requests": rs["requests"], "throughput": round(rs["throughput"], 2),
                          "service_p50_ms": ms(rs["service_p50_s"]), "service_p99_ms": ms(rs["service_p99_s"]),
                          "feasible_fraction_fp64": rs["feasible_fraction"]}

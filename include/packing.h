@@ -147,7 +147,8 @@ int pk_lut_overlap(pk_geometry_t g, pk_lut_t lut, const float* xyt, const int32_
  * evaluates the cost and starts the run).  One call advances every listed slot by up to
  * `iters` iterations; a slot stops for good when it converges or reaches max_iter, with
  * the reference's tests applied to it alone (batch-independent).  Workgroup b runs slot
- * order[b] (order NULL: b).  status_out[s] = 1 once slot s is finished.  History row r of a
+ * order[b] (order NULL: b).  status_out[s] = 1 once slot s is finished, -1 when its ntree
+ * exceeds n_tree_max (the slot is not run).  History row r of a
  * slot is ring slot r (the record holds the count and the oldest slot), the layout every
  * kernel choice reads and writes.
  */

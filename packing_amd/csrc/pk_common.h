@@ -52,6 +52,12 @@ struct GeomT {
     T px[MAXP * MAXV], py[MAXP * MAXV];   // local piece vertices
     T ccx[MAXP], ccy[MAXP], cr[MAXP];     // local piece bounding circles
     T ox[MAXO], oy[MAXO];                 // outline vertices, padded to MAXO with the last
+    // Distinct piece vertices: pieces of a decomposition share most of their vertices
+    // (and padding repeats one), so the evaluators rotate and store each distinct vertex
+    // once per tree.  Vertex v of piece p is distinct vertex (pv[p] >> 8 v) & 0xff.
+    int NU;
+    uint32_t pv[MAXP];
+    T ux[MAXP * MAXV], uy[MAXP * MAXV];
 };
 
 // Host handle: the same polygon in both precisions.

@@ -149,8 +149,8 @@ struct EvalLds {
     T* t;
     T* c;          // [N]   cos, sin of theta
     T* s;
-    T* vx;         // [N*P*MAXV] piece vertices rotated about the tree centre
-    T* vy;
+    T* v;          // [N*NV*2] distinct piece vertices rotated about the tree centre,
+                   // (x, y) interleaved: vertex u of tree i at v[2 (i NV + u)]
     int* head;     // [H]   grid: first tree of each bucket (-1: empty)
     int* next;     // [N]   grid: next tree in the same bucket
     uint32_t* cell;   // [N] grid: biased cell coordinates cx | cy << 16
@@ -161,7 +161,7 @@ struct EvalLds {
                    // cutoff plus a skin at the last rebuild; wcnt[8] holds its length
     int* wcnt;     // [16] per-wave ballot counts; [8] neighbour-list length
     double* wsum;  // [MAX_WAVES] per-wave cost partials (double precision)
-    int N, P, H, nb_cap;
+    int N, P, NV, H, nb_cap;
 };
 
 // Neighbour-list capacity for N trees (pairs).
@@ -169,17 +169,18 @@ __host__ __device__ inline int nb_capacity(int N) { return 6 * N + 64; }
 
 __host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// Bytes of dynamic LDS for one instance of N trees with P pieces (P = 0: no vertices).
-// `periodic` selects the block-wide lists of the periodic-image path instead of the grid.
+// Bytes of dynamic LDS for one instance of N trees with NV distinct piece vertices per
+// tree (NV = 0: no vertices).  `periodic` selects the block-wide lists of the
+// periodic-image path instead of the grid.
 template <typename T>
-__host__ __device__ inline size_t eval_lds_bytes(int N, int P, bool periodic = false, int waves = MAX_WAVES,
+__host__ __device__ inline size_t eval_lds_bytes(int N, int NV, bool periodic = false, int waves = MAX_WAVES,
                                                  bool list = false) {
     size_t b = 0;
     b += sizeof(typename Acc<T>::type) * (3 * (size_t)N + 1);
     b = align_up(b, 16);
     b += sizeof(T) * 5 * (size_t)N;
     b = align_up(b, 16);
-    b += sizeof(T) * 2 * (size_t)N * P * MAXV;
+    b += sizeof(T) * 2 * (size_t)N * NV;
     b = align_up(b, 16);
     if (periodic) {
         b += sizeof(uint32_t) * (TP_CAP + PP_CAP);
@@ -195,7 +196,7 @@ __host__ __device__ inline size_t eval_lds_bytes(int N, int P, bool periodic = f
 }
 
 template <typename T>
-__device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P, bool periodic = false,
+__device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int NV, bool periodic = false,
                                             int waves = MAX_WAVES, bool list = false) {
     EvalLds<T> L;
     size_t o = 0;
@@ -207,9 +208,8 @@ __device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P, bool perio
     L.x = f; L.y = f + N; L.t = f + 2 * N; L.c = f + 3 * N; L.s = f + 4 * N;
     o += sizeof(T) * 5 * (size_t)N;
     o = align_up(o, 16);
-    L.vx = reinterpret_cast<T*>(base + o);
-    L.vy = L.vx + (size_t)N * P * MAXV;
-    o += sizeof(T) * 2 * (size_t)N * P * MAXV;
+    L.v = reinterpret_cast<T*>(base + o);
+    o += sizeof(T) * 2 * (size_t)N * NV;
     o = align_up(o, 16);
     L.H = grid_buckets(N);
     if (periodic) {
@@ -232,7 +232,7 @@ __device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int P, bool perio
         L.tp = L.pp = nullptr;
     }
     L.wsum = reinterpret_cast<double*>(L.wcnt + 16);
-    L.N = N; L.P = P;
+    L.N = N; L.NV = NV; L.P = 0;
     return L;
 }
 
@@ -256,53 +256,53 @@ __host__ __device__ inline void clip_edges(const T (&px)[MAXV], const T (&py)[MA
                                   const T (&qy)[MAXV], bool keep_same, T (&t0)[MAXV],
                                   T (&t1)[MAXV]) {
 #pragma clang fp contract(off)
-    T qex[MAXV], qey[MAXV];
-    double dqx[MAXV], dqy[MAXV], dex[MAXV], dey[MAXV];
-    bool qdeg[MAXV];
-#pragma unroll
-    for (int m = 0; m < MAXV; ++m) {
-        const int m1 = (m + 1) % MAXV;
-        qex[m] = qx[m1] - qx[m];
-        qey[m] = qy[m1] - qy[m];
-        dqx[m] = (double)qx[m];
-        dqy[m] = (double)qy[m];
-        dex[m] = (double)qx[m1] - dqx[m];
-        dey[m] = (double)qy[m1] - dqy[m];
-        qdeg[m] = (dex[m] == 0.0) & (dey[m] == 0.0);
-    }
-    T f[MAXV][MAXV];
-#pragma unroll
-    for (int v = 0; v < MAXV; ++v) {
-        const double pxd = (double)px[v], pyd = (double)py[v];
-#pragma unroll
-        for (int m = 0; m < MAXV; ++m) {
-            const double a = dex[m] * (pyd - dqy[m]);
-            const double b = dey[m] * (pxd - dqx[m]);
-            const double val = a - b;
-            f[v][m] = qdeg[m] ? T(1) : (T)val;
-        }
-    }
+    // Half-plane by half-plane (m outer): only one half-plane's double line and the four
+    // side values of p's vertices against it are live at a time.  Each edge's bounds see
+    // the half-planes in the order m = 0, 1, ..., as an edge-outer loop would.
+    double pxd[MAXV], pyd[MAXV];
+    T ex[MAXV], ey[MAXV];
 #pragma unroll
     for (int k = 0; k < MAXV; ++k) {
         const int k1 = (k + 1) % MAXV;
-        const T ex = px[k1] - px[k], ey = py[k1] - py[k];
-        T lo = T(0);
-        T hi = ((ex == T(0)) & (ey == T(0))) ? T(0) : T(1);
+        pxd[k] = (double)px[k];
+        pyd[k] = (double)py[k];
+        ex[k] = px[k1] - px[k];
+        ey[k] = py[k1] - py[k];
+        t0[k] = T(0);
+        t1[k] = ((ex[k] == T(0)) & (ey[k] == T(0))) ? T(0) : T(1);
+    }
 #pragma unroll
-        for (int m = 0; m < MAXV; ++m) {
-            const T f0 = f[k][m], f1 = f[k1][m];
+    for (int m = 0; m < MAXV; ++m) {
+        const int m1 = (m + 1) % MAXV;
+        const T qex = qx[m1] - qx[m], qey = qy[m1] - qy[m];
+        const double dqx = (double)qx[m], dqy = (double)qy[m];
+        const double dex = (double)qx[m1] - dqx, dey = (double)qy[m1] - dqy;
+        const bool qdeg = (dex == 0.0) & (dey == 0.0);
+        T f[MAXV];
+#pragma unroll
+        for (int v = 0; v < MAXV; ++v) {
+            const double a = dex * (pyd[v] - dqy);
+            const double b = dey * (pxd[v] - dqx);
+            const double val = a - b;
+            f[v] = qdeg ? T(1) : (T)val;
+        }
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k) {
+            const int k1 = (k + 1) % MAXV;
+            const T f0 = f[k], f1 = f[k1];
             const bool coll = (f0 == T(0)) & (f1 == T(0));
-            const bool same = keep_same & (ex * qex[m] + ey * qey[m] > T(0));
+            const bool same = keep_same & (ex[k] * qex + ey[k] * qey > T(0));
             const bool out = ((f0 < T(0)) & (f1 < T(0))) | (coll & !same);
             const T tt = f0 * fast_rcp(f0 - f1);
             const bool leave = (f0 >= T(0)) & (f1 < T(0));
             const bool enter = (f0 < T(0)) & (f1 >= T(0));
+            T hi = t1[k], lo = t0[k];
             hi = leave ? fmin(hi, tt) : hi;
             lo = enter ? fmax(lo, tt) : lo;
             hi = out ? T(-1) : hi;
+            t0[k] = lo;
+            t1[k] = hi;
         }
-        t0[k] = lo;
-        t1[k] = hi;
     }
 }
 
@@ -537,13 +537,16 @@ struct Evaluator {
     // relative to tree i; cost to this lane, gradient to both trees.
     __device__ inline void piece_pair(int i, int j, int pi, int pj, T dx, T dy, bool self) {
         T ax[MAXV], ay[MAXV], bx[MAXV], by[MAXV];
-        const int oa = (i * L.P + pi) * MAXV, ob = (j * L.P + pj) * MAXV;
+        const uint32_t wa = G.pv[pi], wb = G.pv[pj];
+        const T* va = L.v + 2 * i * L.NV;
+        const T* vb = L.v + 2 * j * L.NV;
 #pragma unroll
         for (int v = 0; v < MAXV; ++v) {
-            ax[v] = L.vx[oa + v];
-            ay[v] = L.vy[oa + v];
-            bx[v] = L.vx[ob + v] + dx;
-            by[v] = L.vy[ob + v] + dy;
+            const int ua = (wa >> (8 * v)) & 0xff, ub = (wb >> (8 * v)) & 0xff;
+            ax[v] = va[2 * ua];
+            ay[v] = va[2 * ua + 1];
+            bx[v] = vb[2 * ub] + dx;
+            by[v] = vb[2 * ub + 1] + dy;
         }
         T gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0};
         T c;
@@ -660,12 +663,15 @@ struct Evaluator {
     // so it contributes exactly zero and needs no clipping.
     __device__ inline bool centre_axis_separated(int i, int j, int pi, int pj, T dx, T dy, T ddx,
                                                  T ddy) const {
-        const int oa = (i * L.P + pi) * MAXV, ob = (j * L.P + pj) * MAXV;
+        const uint32_t wa = G.pv[pi], wb = G.pv[pj];
+        const T* va = L.v + 2 * i * L.NV;
+        const T* vb = L.v + 2 * j * L.NV;
         T amax = -T(1e30), bmin = T(1e30);
 #pragma unroll
         for (int v = 0; v < MAXV; ++v) {
-            amax = fmax(amax, L.vx[oa + v] * ddx + L.vy[oa + v] * ddy);
-            bmin = fmin(bmin, (L.vx[ob + v] + dx) * ddx + (L.vy[ob + v] + dy) * ddy);
+            const int ua = (wa >> (8 * v)) & 0xff, ub = (wb >> (8 * v)) & 0xff;
+            amax = fmax(amax, va[2 * ua] * ddx + va[2 * ua + 1] * ddy);
+            bmin = fmin(bmin, (vb[2 * ub] + dx) * ddx + (vb[2 * ub + 1] + dy) * ddy);
         }
         const T gap = bmin - amax, mu = T(1e-4) * G.radius;
         return gap > T(0) && gap * gap > mu * mu * (ddx * ddx + ddy * ddy);
@@ -974,7 +980,7 @@ struct Evaluator {
 // cell grid (GRID); clear the accumulators.  Ends with a barrier.
 template <typename T, int BLOCK, bool VERTS = true, bool GRID = true>
 __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
-    const int N = L.N, P = L.P;
+    const int N = L.N, NV = L.NV;
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         T sn, cs;
         sincos_t(L.t[i], &sn, &cs);
@@ -986,13 +992,13 @@ __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
         for (int h = threadIdx.x; h < L.H; h += BLOCK) L.head[h] = -1;
     __syncthreads();
     if (VERTS) {
-        const int nv = N * P * MAXV;
+        const int nv = N * NV;
         for (int q = threadIdx.x; q < nv; q += BLOCK) {
-            const int i = q / (P * MAXV);
-            const int r = q - i * P * MAXV;
-            const T lx = G.px[r], ly = G.py[r];
-            L.vx[q] = L.c[i] * lx - L.s[i] * ly;
-            L.vy[q] = L.s[i] * lx + L.c[i] * ly;
+            const int i = q / NV;
+            const int r = q - i * NV;
+            const T lx = G.ux[r], ly = G.uy[r];
+            L.v[2 * q] = L.c[i] * lx - L.s[i] * ly;
+            L.v[2 * q + 1] = L.s[i] * lx + L.c[i] * ly;
         }
     }
     if (GRID) {

@@ -40,7 +40,8 @@ __global__ __launch_bounds__(BLOCK) void k_overlap(const T* __restrict__ xyt,
     }
     load_geom(G, geom);
     __syncthreads();
-    EvalLds<T> L = carve_eval_lds<T>(smem, N, G.P, CRYSTAL, BLOCK / WAVE);
+    EvalLds<T> L = carve_eval_lds<T>(smem, N, G.NU, CRYSTAL, BLOCK / WAVE);
+    L.P = G.P;
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         L.x[i] = xyt[3 * (off + i) + 0];
         L.y[i] = xyt[3 * (off + i) + 1];
@@ -163,8 +164,8 @@ static int run_overlap(const Geometry* g, const void* xyt, const int* inst_off, 
                        int n_trees, const void* crystal, int npd, int only_self, void* cost,
                        void* grads, unsigned long long* stats, hipStream_t st) {
     auto kern = k_overlap<T, MODE, GRAD, CRYSTAL, BLOCK>;
-    const int P = (sizeof(T) == 4) ? g->hf.P : g->hd.P;
-    const size_t lds = eval_lds_bytes<T>(n_trees, P, CRYSTAL, BLOCK / WAVE);
+    const int NV = (sizeof(T) == 4) ? g->hf.NU : g->hd.NU;
+    const size_t lds = eval_lds_bytes<T>(n_trees, NV, CRYSTAL, BLOCK / WAVE);
     int rc = launch_lds(kern, n_ens, BLOCK, lds, st, sizeof(GeomT<T>));
     if (rc) return rc;
     const GeomT<T>* dg = reinterpret_cast<const GeomT<T>*>(sizeof(T) == 4 ? (const void*)g->df
@@ -256,6 +257,26 @@ int pk_geometry_create(const double* piece_xy, const int32_t* piece_nverts, int 
         g->hd.ccx[p] = cx; g->hd.ccy[p] = cy; g->hd.cr[p] = r;
         g->hf.ccx[p] = (float)cx; g->hf.ccy[p] = (float)cy; g->hf.cr[p] = (float)r;
     }
+    // distinct piece vertices (exact equality of the double coordinates)
+    int nu = 0;
+    for (int p = 0; p < n_pieces; ++p) {
+        uint32_t word = 0;
+        for (int v = 0; v < MAXV; ++v) {
+            const double x = g->hd.px[p * MAXV + v], y = g->hd.py[p * MAXV + v];
+            int u = 0;
+            while (u < nu && !(g->hd.ux[u] == x && g->hd.uy[u] == y)) ++u;
+            if (u == nu) {
+                g->hd.ux[nu] = x;
+                g->hd.uy[nu] = y;
+                g->hf.ux[nu] = (float)x;
+                g->hf.uy[nu] = (float)y;
+                ++nu;
+            }
+            word |= (uint32_t)u << (8 * v);
+        }
+        g->hd.pv[p] = g->hf.pv[p] = word;
+    }
+    g->hd.NU = g->hf.NU = nu;
     // the outline padded to MAXO by repeating its last vertex (boundary_tree reads whole
     // blocks of 8; a repeat never wins its strict arg-max)
     for (int v = 0; v < MAXO; ++v) {

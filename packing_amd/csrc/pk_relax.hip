@@ -86,13 +86,13 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     if (threadIdx.x == 0 && list_mode == 1) L.wcnt[8] = 0;
     __syncthreads();
     if (MODE != MODE_LUT) {
-        const int P = L.P, nv = N * P * MAXV;
+        const int NV = L.NV, nv = N * NV;
         for (int q = threadIdx.x; q < nv; q += BLOCK) {
-            const int i = q / (P * MAXV);
-            const int r = q - i * P * MAXV;
-            const float lx = G.px[r], ly = G.py[r];
-            L.vx[q] = L.c[i] * lx - L.s[i] * ly;
-            L.vy[q] = L.s[i] * lx + L.c[i] * ly;
+            const int i = q / NV;
+            const int r = q - i * NV;
+            const float lx = G.ux[r], ly = G.uy[r];
+            L.v[2 * q] = L.c[i] * lx - L.s[i] * ly;
+            L.v[2 * q + 1] = L.s[i] * lx + L.c[i] * ly;
         }
     }
     if (grid) {
@@ -178,8 +178,8 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
 // Dynamic LDS of one workgroup: evaluator, x g g_prev d, the small L-BFGS matrices S^T Y and
 // Y^T Y (history x history doubles each), and (hl) the history s, y; the vectors padded to a
 // multiple of 64 floats.
-size_t relax_lds_bytes(int mode, int n_max, int P, int m, bool hl, int block) {
-    const int Pe = mode == MODE_LUT ? 0 : P;
+size_t relax_lds_bytes(int mode, int n_max, int NV, int m, bool hl, int block) {
+    const int Pe = mode == MODE_LUT ? 0 : NV;
     size_t b = eval_lds_bytes<float>(n_max, Pe, false, block / WAVE, true);
     const size_t np = (size_t)pad64(3 * n_max + 3);
     b += sizeof(float) * 4 * np;
@@ -531,7 +531,12 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     if (b >= p.n_launch) return;
     const int slot = p.order ? p.order[b] : b;
     const int N = p.ntree[slot];
-    if (N <= 0) return;
+    // LDS, the row count and the tile were sized for n_tree_max: a larger count is refused
+    // (status -1) rather than run past them
+    if (N <= 0 || N > p.n_tree_max) {
+        if (N > 0 && threadIdx.x == 0 && p.status_out) p.status_out[slot] = -1;
+        return;
+    }
     LbSlot* gS = reinterpret_cast<LbSlot*>(p.state) + slot;
     if (threadIdx.x == 0) S = *gS;
     load_geom(G, A.geom);
@@ -545,9 +550,9 @@ __global__ __launch_bounds__(BLOCK) void k_relax(RelaxArgs A) {
     float* __restrict__ Sh = p.s_hist + (long long)m * xoff;
     float* __restrict__ Yh = p.y_hist + (long long)m * xoff;
 
-    EvalLds<float> L = carve_eval_lds<float>(smem, N, MODE == MODE_LUT ? 0 : G.P, false, BLOCK / WAVE, true);
+    EvalLds<float> L = carve_eval_lds<float>(smem, N, MODE == MODE_LUT ? 0 : G.NU, false, BLOCK / WAVE, true);
     L.P = G.P;
-    char* vb = smem + eval_lds_bytes<float>(N, MODE == MODE_LUT ? 0 : G.P, false, BLOCK / WAVE, true);
+    char* vb = smem + eval_lds_bytes<float>(N, MODE == MODE_LUT ? 0 : G.NU, false, BLOCK / WAVE, true);
     float* x = reinterpret_cast<float*>(vb);   // [np] each, zero beyond n
     float* g = x + np;
     float* gp = g + np;
@@ -875,7 +880,12 @@ __global__ __launch_bounds__(WAVE * NWV, (kernel_occupancy<MODE, R, NWV>())) voi
     if (b >= p.n_launch) return;
     const int slot = p.order ? p.order[b] : b;
     const int N = p.ntree[slot];
-    if (N <= 0) return;
+    // LDS, the row count and the tile were sized for n_tree_max: a larger count is refused
+    // (status -1) rather than run past them
+    if (N <= 0 || N > p.n_tree_max) {
+        if (N > 0 && threadIdx.x == 0 && p.status_out) p.status_out[slot] = -1;
+        return;
+    }
     LbSlot* gS = reinterpret_cast<LbSlot*>(p.state) + slot;
     const int lane = NWV == 1 ? (int)threadIdx.x : (int)threadIdx.x % WAVE;
     const int wave = NWV == 1 ? 0 : (int)threadIdx.x / WAVE;
@@ -893,7 +903,7 @@ __global__ __launch_bounds__(WAVE * NWV, (kernel_occupancy<MODE, R, NWV>())) voi
     float* __restrict__ Sh = p.s_hist + (long long)m * xoff;
     float* __restrict__ Yh = p.y_hist + (long long)m * xoff;
 
-    const int Pe = MODE == MODE_LUT ? 0 : G.P;
+    const int Pe = MODE == MODE_LUT ? 0 : G.NU;
     EvalLds<float> L = carve_eval_lds<float>(smem, N, Pe, false, NWV, true);
     L.P = G.P;
     char* vb = smem + eval_lds_bytes<float>(N, Pe, false, NWV, true);
@@ -1263,8 +1273,8 @@ __global__ __launch_bounds__(WAVE * NWV, (kernel_occupancy<MODE, R, NWV>())) voi
 
 // Dynamic LDS of k_relax_w: evaluator (NWV waves), evaluator input/output vectors, small
 // matrices, and (two waves) the exchange buffers.
-size_t relax_w_lds_bytes(int mode, int n_max, int P, int m, int R, int nwv) {
-    const int Pe = mode == MODE_LUT ? 0 : P;
+size_t relax_w_lds_bytes(int mode, int n_max, int NV, int m, int R, int nwv) {
+    const int Pe = mode == MODE_LUT ? 0 : NV;
     const int rt = nwv == 2 ? 2 * ROWS_A : R;
     size_t b = eval_lds_bytes<float>(n_max, Pe, false, nwv, true);
     b += sizeof(float) * 2 * WAVE * (size_t)rt;
@@ -1293,9 +1303,9 @@ static int launch_relax(const Geometry* g, const RelaxArgs& A, hipStream_t st) {
     int lim = 0;
     PK_CHECK_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     const size_t stat = sizeof(GeomT<float>) + sizeof(LbSlot) + 256;
-    const int P = g->hf.P, m = A.p.history, nmax = A.p.n_tree_max;
-    const size_t lds_g = relax_lds_bytes(MODE, nmax, P, m, false, BLOCK);
-    const size_t lds_l = relax_lds_bytes(MODE, nmax, P, m, true, BLOCK);
+    const int NV = g->hf.NU, m = A.p.history, nmax = A.p.n_tree_max;
+    const size_t lds_g = relax_lds_bytes(MODE, nmax, NV, m, false, BLOCK);
+    const size_t lds_l = relax_lds_bytes(MODE, nmax, NV, m, true, BLOCK);
     const bool fits_l = lds_l + stat <= (size_t)lim;
     if (A.p.hist_lds == 1 && !fits_l) {
         set_error("pk_relax_step: history of " + std::to_string(nmax) + " trees does not fit in LDS");
@@ -1321,7 +1331,7 @@ static int launch_relax_w_rk(const Geometry* g, const RelaxArgs& A, hipStream_t 
     PK_CHECK_HIP(hipGetDevice(&dev));
     int lim = 0;
     PK_CHECK_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t lds = relax_w_lds_bytes(MODE, A.p.n_tree_max, g->hf.P, A.p.history, R, NWV);
+    const size_t lds = relax_w_lds_bytes(MODE, A.p.n_tree_max, g->hf.NU, A.p.history, R, NWV);
     const size_t stat = sizeof(GeomT<float>) + sizeof(LbSlot) + 64;
     if (lds + stat > (size_t)lim) {
         set_error("pk_relax_step: instance of " + std::to_string(A.p.n_tree_max) + " trees needs " +
@@ -1456,9 +1466,9 @@ int pk_relax_lds_bytes(pk_geometry_t h, int mode, int n_tree_max, int history, i
     const Geometry* g = reinterpret_cast<const Geometry*>(h);
     if (!g || n_tree_max < 1 || history < 1) return PK_EINVAL;
     if (block == 1 || block == 2)
-        return (int)relax_w_lds_bytes(mode, n_tree_max, g->hf.P, history, wave_rows(n_tree_max), block);
+        return (int)relax_w_lds_bytes(mode, n_tree_max, g->hf.NU, history, wave_rows(n_tree_max), block);
     if (block != 64 && block != 128 && block != 256) return PK_EINVAL;
-    return (int)relax_lds_bytes(mode, n_tree_max, g->hf.P, history, hist_lds != 0, block);
+    return (int)relax_lds_bytes(mode, n_tree_max, g->hf.NU, history, hist_lds != 0, block);
 }
 
 int pk_relax_step(pk_geometry_t h, const pk_relax_params* p, void* stream) {

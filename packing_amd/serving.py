@@ -121,7 +121,9 @@ class RelaxServer:
         # host staging of admitted poses (slot s at s * stride) and of the per-step lists
         self.x_host = torch.zeros(n_slots * n, dtype=torch.float32).pin_memory()
         self.order_h = torch.zeros((self.in_flight + 1) * n_slots, dtype=torch.int32).pin_memory()
-        self.adm_h = torch.zeros(2 * n_slots, dtype=torch.int64).pin_memory()
+        # one admission buffer per step that can be queued: a step's non-blocking copy reads
+        # its buffer when the stream reaches it, so the next step must not overwrite it
+        self.adm_h = torch.zeros((self.in_flight + 1, 2 * n_slots), dtype=torch.int64).pin_memory()
         self.status_h = [torch.zeros(n_slots, dtype=torch.int32).pin_memory() for _ in range(self.in_flight + 1)]
         self.stream = torch.cuda.Stream(device=dev)
         self.side = [torch.cuda.Stream(device=dev) for _ in range(len(R_CLASS_MAX))]
@@ -163,12 +165,13 @@ class RelaxServer:
         if not self.pending:
             return
         k = len(self.pending)
-        a = self.adm_h.numpy()
+        buf = self.adm_h[self._k % (self.in_flight + 1)]
+        a = buf.numpy()
         a[:k] = self.pending
         a[k:2 * k] = [self.busy[s].xyt.shape[0] for s in self.pending]
         self.pending = []
         with torch.cuda.stream(self.stream):
-            d = self.adm_h[:2 * k].to(self.dev, non_blocking=True)
+            d = buf[:2 * k].to(self.dev, non_blocking=True)
             idx, nt = d[:k], d[k:].to(torch.int32)
             self.state.view(self.S, -1).index_fill_(0, idx, 0)
             self.status.index_fill_(0, idx, 0)
@@ -251,6 +254,9 @@ class RelaxServer:
         """Retire the slots that the finished step left done: results and feasibility on the
         retire stream (the next step keeps running), then back to the host."""
         st = step.status_h.numpy()
+        bad = [s for s in step.slots if st[s] < 0]
+        if bad:
+            raise RuntimeError(f"relaxation refused slots {bad[:8]}: tree count above the launch's bound")
         fin = [s for s, r in step.slots.items() if st[s] == 1 and self.busy.get(s) is r]
         if not fin:
             return

@@ -86,3 +86,58 @@ def test_mixed_sizes_saturation_and_one_step_in_flight():
         b.run()
         a = b.xyt(0).cpu()
         assert torch.equal(a, out[2][r.rid]) and torch.equal(a, out[1][r.rid]), r.rid
+
+
+def test_admission_buffers_survive_the_next_step():
+    """Each step's admitted slots and tree counts are staged in a host buffer of their own:
+    the step's non-blocking copy may run after the host has queued the next step, so a
+    shared buffer could hand it the next step's slots (and slot ids as tree counts)."""
+    from packing_amd.context import set_geometry
+    from packing_amd.relax import RelaxOptions
+    from packing_amd.serving import RelaxServer, poisson_requests
+    set_geometry(GEOMS["tree"]())
+    opts = RelaxOptions(mode="area", max_iter=20, history=4, max_step=0.1)
+    srv = RelaxServer(opts, n_slots=8, n_max=40, chunk=5, in_flight=2)
+    reqs = poisson_requests(6, rate=1e6, n_lo=10, n_hi=40)
+    srv._admit(reqs[0], 0.0)
+    srv._admit(reqs[1], 0.0)
+    srv._launch()
+    first = srv.adm_h[0, :4].tolist()
+    srv._admit(reqs[2], 0.0)
+    srv._launch()
+    # the first step's staging is untouched by the second step's admission
+    assert srv.adm_h[0, :4].tolist() == first
+    assert srv.adm_h[1, 0].item() == reqs[2].slot and srv.adm_h[1, 1].item() == reqs[2].xyt.shape[0]
+    torch.cuda.synchronize()
+    nt = srv.ntree.cpu().numpy()
+    for r in reqs[:3]:
+        assert nt[r.slot] == r.xyt.shape[0]
+
+
+def test_step_refuses_tree_count_above_bound():
+    """A slot whose tree count exceeds the launch's n_tree_max is not run: status -1, its
+    state untouched (the kernel's LDS and rows are sized for n_tree_max)."""
+    import ctypes as C
+    from packing_amd import _native as N
+    from packing_amd.context import device_geometry, set_geometry
+    from packing_amd.relax import RelaxOptions
+    from packing_amd.serving import RelaxServer, poisson_requests
+    geom = GEOMS["tree"]()
+    set_geometry(geom)
+    opts = RelaxOptions(mode="area", max_iter=20, history=4, max_step=0.1)
+    srv = RelaxServer(opts, n_slots=4, n_max=40, chunk=5, in_flight=1)
+    reqs = poisson_requests(2, rate=1e6, n_lo=30, n_hi=40)
+    for r in reqs:
+        srv._admit(r, 0.0)
+    srv._reset_pending()
+    torch.cuda.synchronize()
+    x0 = srv.x.clone()
+    order = torch.tensor([r.slot for r in reqs], dtype=torch.int32, device=srv.x.device)
+    p = srv._params(order.data_ptr(), 2, 20)   # bound below both tree counts
+    st = torch.cuda.current_stream()
+    N.check(N.lib().pk_relax_step(device_geometry(srv.x.device, geom).handle, C.byref(p), st.cuda_stream),
+            "pk_relax_step")
+    torch.cuda.synchronize()
+    s = srv.status.cpu().numpy()
+    assert all(s[r.slot] == -1 for r in reqs)
+    assert torch.equal(srv.x, x0)
