@@ -303,6 +303,12 @@ __host__ __device__ inline void clip_edges(const T (&px)[MAXV], const T (&py)[MA
             t0[k] = lo;
             t1[k] = hi;
         }
+#ifdef __HIP_DEVICE_COMPILE__
+        // one half-plane at a time: interleaving them buys no latency the other waves do
+        // not hide, and holds enough values to push the exact kernels past 128 registers
+        // (three waves per SIMD instead of four)
+        __builtin_amdgcn_sched_barrier(0);
+#endif
     }
 }
 
@@ -678,41 +684,50 @@ struct Evaluator {
     }
 
     // Expand the tn tree pairs of tq into the piece pairs whose bounding circles meet,
-    // clipping 64 at a time from the wave's piece-pair queue.
+    // clipping 64 at a time from the wave's piece-pair queue.  One lane per (tree pair,
+    // piece pi of tree i): the pair's poses and piece pi's circle are read once per run of
+    // pieces pj of tree j (the same pj on every lane: broadcast reads); a run ends when the
+    // queue holds 64 pairs, which are clipped before the lane state is read again (so that
+    // no lane state is live across the clip).
     __device__ inline void expand(const uint32_t* tq, int tn, uint32_t* pq, int& pn) {
         const int lane = threadIdx.x % WAVE;
-        const int P = L.P, PP = P * P;
-        const float iPP = 1.f / (float)PP, iP = 1.f / (float)P;
-        const int items = tn * PP;
+        const int P = L.P;
+        const float iP = 1.f / (float)P;
+        const int items = tn * P;
         for (int u0 = 0; u0 < items; u0 += WAVE) {
             const int u = u0 + lane;
-            bool pred = false;
-            uint32_t entry = 0;
-            if (u < items) {
-                // exact: u, PP < 2^15, and (u + 1/2) / PP is at least 1/(2 PP) from an integer
-                const int q = (int)(((float)u + 0.5f) * iPP);
-                const int r = u - q * PP;
-                const int pi = (int)(((float)r + 0.5f) * iP);
-                const int pj = r - pi * P;
-                const uint32_t e = tq[q];
-                const int i = e & 0xfff, j = (e >> 12) & 0xfff;
-                const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-                const T cxa = L.c[i] * G.ccx[pi] - L.s[i] * G.ccy[pi];
-                const T cya = L.s[i] * G.ccx[pi] + L.c[i] * G.ccy[pi];
-                const T cxb = L.c[j] * G.ccx[pj] - L.s[j] * G.ccy[pj] + dx;
-                const T cyb = L.s[j] * G.ccx[pj] + L.c[j] * G.ccy[pj] + dy;
-                const T rr = G.cr[pi] + G.cr[pj];
-                const T ddx = cxb - cxa, ddy = cyb - cya;
-                pred = ddx * ddx + ddy * ddy <= rr * rr;
-                if (pred) pred = !centre_axis_separated(i, j, pi, pj, dx, dy, ddx, ddy);
-                entry = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
-            }
-            const uint64_t m = __ballot(pred);
-            if (m) {
-                if (pred) pq[pn + __popcll(m & lanemask_lt())] = entry;
-                pn += __popcll(m);
-                wave_sync();
+            const bool act = u < items;
+            int pj = 0;
+            while (pj < P) {
+                {
+                    // exact: u < 2^15 and (u + 1/2) / P is at least 1/(2 P) from an integer
+                    const int q = act ? (int)(((float)u + 0.5f) * iP) : 0;
+                    const int pi = act ? u - q * P : 0;
+                    const uint32_t e = tq[q];
+                    const int i = e & 0xfff, j = (e >> 12) & 0xfff;
+                    const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                    const T ci = L.c[i], si = L.s[i], cj = L.c[j], sj = L.s[j];
+                    const T cxa = ci * G.ccx[pi] - si * G.ccy[pi];
+                    const T cya = si * G.ccx[pi] + ci * G.ccy[pi];
+                    const T ra = G.cr[pi];
+                    for (; pj < P;) {
+                        const T bx = G.ccx[pj], by = G.ccy[pj];
+                        const T cxb = cj * bx - sj * by + dx;
+                        const T cyb = sj * bx + cj * by + dy;
+                        const T rr = ra + G.cr[pj];
+                        const T ddx = cxb - cxa, ddy = cyb - cya;
+                        bool pred = act && ddx * ddx + ddy * ddy <= rr * rr;
+                        if (pred) pred = !centre_axis_separated(i, j, pi, pj, dx, dy, ddx, ddy);
+                        const uint64_t m = __ballot(pred);
+                        if (pred)
+                            pq[pn + __popcll(m & lanemask_lt())] = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
+                        pn += __popcll(m);
+                        ++pj;
+                        if (pn >= WAVE) break;
+                    }
+                }
                 if (pn >= WAVE) {
+                    wave_sync();
                     clip_entry(pq[pn - WAVE + lane]);
                     pn -= WAVE;
                     n_pp += WAVE;
@@ -720,6 +735,7 @@ struct Evaluator {
                 }
             }
         }
+        wave_sync();
     }
 
     // Ballot-append a candidate to the wave's tree-pair queue; drain 64 entries when it
