@@ -77,7 +77,9 @@ def overlap_kernel_roofline(geom, dev, n_ens=1024, n_trees=100, reps=10, seed=1)
     ms = e0.elapsed_time(e1) / reps
     fl = area_eval_flops(n_ens * n_trees, geom.n_pieces, tp, pp)
     tf = fl / (ms * 1e-3) / 1e12
-    return {"kernel": "k_overlap<float, area, grad>", "bound": "valu",
+    traffic, src = profiled_traffic("pk::k_overlap<float, 0, true, false, 256>")
+    return {"kernel": "k_overlap<float, area, grad>", "bound": "valu", "traffic": traffic,
+            "traffic_source": src,
             "workload": f"{n_ens} instances x {n_trees} trees, initial poses, random 15-vertex "
                         f"polygon ({geom.n_pieces} pieces)",
             "avg_launch_ms": round(ms, 4), "tree_pairs": tp, "piece_pairs": pp,
@@ -216,7 +218,7 @@ def bucket_roofline(b, sizes, opts, chunk, n_chunks, stream):
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 5), "traffic": traffic,
             "traffic_source": src,
-            # share of the wave's cycles spent issuing instructions (one wave per SIMD):
+            # share of the wave's cycles spent issuing instructions (two waves per SIMD):
             # what actually bounds this kernel, from the same profile
             "issue_frac": issue,
             "kernel": dom["kernel"] + f" (table mode, size bucket of {dom['instances']} instances, "
