@@ -124,6 +124,8 @@ __device__ inline void load_geom(GeomT<T>& G, const GeomT<T>* __restrict__ src) 
 constexpr int TP_CAP = 1024;   // block-wide tree-pair list capacity (periodic images)
 constexpr int PP_CAP = 2048;   // block-wide piece-pair list capacity (periodic images)
 constexpr int QCAP = 160;      // per-wave queue capacity (entries), tree pairs and piece pairs
+constexpr int XQ = 128;        // exact modes: per-wave tree-pair, piece-item and piece-pair
+                               // queues of XQ entries each (each holds at most 127)
 constexpr int LUT_UMAX = 4;    // most table lookups per lane per drain (the queue holds
                                // 2 QCAP >= 64 LUT_UMAX + 63 entries)
 static_assert(2 * QCAP >= WAVE * LUT_UMAX + WAVE - 1, "table queue capacity");
@@ -154,15 +156,19 @@ struct EvalLds {
     int* head;     // [H]   grid: first tree of each bucket (-1: empty)
     int* next;     // [N]   grid: next tree in the same bucket
     uint32_t* cell;   // [N] grid: biased cell coordinates cx | cy << 16
-    uint32_t* wq;  // [waves * 2 * QCAP] per-wave tree-pair and piece-pair queues
+    uint32_t* wq;  // [waves * wqs] per-wave queue space: table mode one queue of 2 QCAP
+                   // entries; exact modes tree pairs, piece items, piece pairs (XQ each)
     uint32_t* tp;  // [TP_CAP] periodic images: tree-pair entries i | j << 12 | cell << 24
     uint32_t* pp;  // [PP_CAP] periodic images: piece-pair entries tp index | pi << 16 | pj << 20
     uint32_t* nb;  // [nb_cap] neighbour list (relaxation): pairs a | b << 12 within the
                    // cutoff plus a skin at the last rebuild; wcnt[8] holds its length
     int* wcnt;     // [16] per-wave ballot counts; [8] neighbour-list length
     double* wsum;  // [MAX_WAVES] per-wave cost partials (double precision)
-    int N, P, NV, H, nb_cap;
+    int N, P, NV, H, nb_cap, wqs;
 };
+
+// Entries of queue space per wave: exact modes (NV > 0) three queues of XQ, table mode 2 QCAP.
+__host__ __device__ inline int wave_queue_entries(int NV) { return NV > 0 ? 3 * XQ : 2 * QCAP; }
 
 // Neighbour-list capacity for N trees (pairs).
 __host__ __device__ inline int nb_capacity(int N) { return 6 * N + 64; }
@@ -187,7 +193,7 @@ __host__ __device__ inline size_t eval_lds_bytes(int N, int NV, bool periodic = 
     } else {
         b += sizeof(int) * ((size_t)grid_buckets(N) + 2 * (size_t)N);
         b = align_up(b, 16);
-        b += sizeof(uint32_t) * waves * 2 * QCAP;
+        b += sizeof(uint32_t) * waves * wave_queue_entries(NV);
         if (list) b += sizeof(uint32_t) * (size_t)nb_capacity(N);
     }
     b += sizeof(int) * 16;
@@ -226,13 +232,13 @@ __device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int NV, bool peri
         o += sizeof(int) * ((size_t)L.H + 2 * (size_t)N);
         o = align_up(o, 16);
         L.wq = reinterpret_cast<uint32_t*>(base + o);
-        L.nb = L.wq + waves * 2 * QCAP;
+        L.nb = L.wq + waves * wave_queue_entries(NV);
         L.nb_cap = list ? nb_capacity(N) : 0;
         L.wcnt = reinterpret_cast<int*>(L.nb + L.nb_cap);
         L.tp = L.pp = nullptr;
     }
     L.wsum = reinterpret_cast<double*>(L.wcnt + 16);
-    L.N = N; L.NV = NV; L.P = 0;
+    L.N = N; L.NV = NV; L.P = 0; L.wqs = wave_queue_entries(NV);
     return L;
 }
 
@@ -683,59 +689,101 @@ struct Evaluator {
         return gap > T(0) && gap * gap > mu * mu * (ddx * ddx + ddy * ddy);
     }
 
-    // Expand the tn tree pairs of tq into the piece pairs whose bounding circles meet,
-    // clipping 64 at a time from the wave's piece-pair queue.  One lane per (tree pair,
-    // piece pi of tree i): the pair's poses and piece pi's circle are read once per run of
-    // pieces pj of tree j (the same pj on every lane: broadcast reads); a run ends when the
+    // Pieces pj of tree j against the items of iq (tree pair q | piece pi of tree i << 16)
+    // held by lanes below cnt: the item's poses and piece pi's circle are read once per run
+    // of pj (the same pj on every lane: broadcast reads); a run ends when the piece-pair
     // queue holds 64 pairs, which are clipped before the lane state is read again (so that
     // no lane state is live across the clip).
+    __device__ __attribute__((always_inline)) inline void expand_items(const uint32_t* tq, const uint32_t* iq,
+                                                                        int cnt, uint32_t* pq, int& pn) {
+        const int lane = threadIdx.x % WAVE;
+        const int P = L.P;
+        const bool act = lane < cnt;
+        const uint32_t it = iq[act ? lane : 0];
+        int pj = 0;
+        while (pj < P) {
+            {
+                const int pi = (int)(it >> 16);
+                const uint32_t e = tq[it & 0xffff];
+                const int i = e & 0xfff, j = (e >> 12) & 0xfff;
+                const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                const T ci = L.c[i], si = L.s[i], cj = L.c[j], sj = L.s[j];
+                const T cxa = ci * G.ccx[pi] - si * G.ccy[pi];
+                const T cya = si * G.ccx[pi] + ci * G.ccy[pi];
+                const T ra = G.cr[pi];
+                for (; pj < P;) {
+                    const T bx = G.ccx[pj], by = G.ccy[pj];
+                    const T cxb = cj * bx - sj * by + dx;
+                    const T cyb = sj * bx + cj * by + dy;
+                    const T rr = ra + G.cr[pj];
+                    const T ddx = cxb - cxa, ddy = cyb - cya;
+                    bool pred = act && ddx * ddx + ddy * ddy <= rr * rr;
+                    if (pred) pred = !centre_axis_separated(i, j, pi, pj, dx, dy, ddx, ddy);
+                    const uint64_t m = __ballot(pred);
+                    if (pred)
+                        pq[pn + __popcll(m & lanemask_lt())] = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
+                    pn += __popcll(m);
+                    ++pj;
+                    if (pn >= WAVE) break;
+                }
+            }
+            if (pn >= WAVE) {
+                wave_sync();
+                clip_entry(pq[pn - WAVE + lane]);
+                pn -= WAVE;
+                n_pp += WAVE;
+                wave_sync();
+            }
+        }
+    }
+
+    // Expand the tn tree pairs of tq into the piece pairs whose bounding circles meet,
+    // clipping 64 at a time from the wave's piece-pair queue.  First one lane per (tree
+    // pair, piece pi of tree i): a piece whose circle misses tree j's bounding circle (which
+    // holds every piece of tree j) meets no piece of j and is dropped; the others are
+    // compacted into the item queue and expanded 64 at a time over the pieces of tree j.
+    // Dropped pieces would clip to exactly zero, so the results are unchanged.
     __device__ inline void expand(const uint32_t* tq, int tn, uint32_t* pq, int& pn) {
         const int lane = threadIdx.x % WAVE;
+        uint32_t* iq = const_cast<uint32_t*>(tq) + XQ;
         const int P = L.P;
         const float iP = 1.f / (float)P;
         const int items = tn * P;
-        for (int u0 = 0; u0 < items; u0 += WAVE) {
+        int in_ = 0, u0 = 0;
+        // one call site of expand_items (the clip is inlined once): each pass tests the next
+        // 64 items, then expands 64 queued items, or the rest once every item is tested
+        while (u0 < items || in_ > 0) {
             const int u = u0 + lane;
-            const bool act = u < items;
-            int pj = 0;
-            while (pj < P) {
-                {
-                    // exact: u < 2^15 and (u + 1/2) / P is at least 1/(2 P) from an integer
-                    const int q = act ? (int)(((float)u + 0.5f) * iP) : 0;
-                    const int pi = act ? u - q * P : 0;
-                    const uint32_t e = tq[q];
-                    const int i = e & 0xfff, j = (e >> 12) & 0xfff;
-                    const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-                    const T ci = L.c[i], si = L.s[i], cj = L.c[j], sj = L.s[j];
-                    const T cxa = ci * G.ccx[pi] - si * G.ccy[pi];
-                    const T cya = si * G.ccx[pi] + ci * G.ccy[pi];
-                    const T ra = G.cr[pi];
-                    for (; pj < P;) {
-                        const T bx = G.ccx[pj], by = G.ccy[pj];
-                        const T cxb = cj * bx - sj * by + dx;
-                        const T cyb = sj * bx + cj * by + dy;
-                        const T rr = ra + G.cr[pj];
-                        const T ddx = cxb - cxa, ddy = cyb - cya;
-                        bool pred = act && ddx * ddx + ddy * ddy <= rr * rr;
-                        if (pred) pred = !centre_axis_separated(i, j, pi, pj, dx, dy, ddx, ddy);
-                        const uint64_t m = __ballot(pred);
-                        if (pred)
-                            pq[pn + __popcll(m & lanemask_lt())] = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
-                        pn += __popcll(m);
-                        ++pj;
-                        if (pn >= WAVE) break;
-                    }
-                }
-                if (pn >= WAVE) {
-                    wave_sync();
-                    clip_entry(pq[pn - WAVE + lane]);
-                    pn -= WAVE;
-                    n_pp += WAVE;
-                    wave_sync();
-                }
+            bool pred = false;
+            uint32_t item = 0;
+            if (u < items) {
+                // exact: u < 2^15 and (u + 1/2) / P is at least 1/(2 P) from an integer
+                const int q = (int)(((float)u + 0.5f) * iP);
+                const int pi = u - q * P;
+                const uint32_t e = tq[q];
+                const int i = e & 0xfff, j = (e >> 12) & 0xfff;
+                const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                const T ci = L.c[i], si = L.s[i];
+                const T ox = ci * G.ccx[pi] - si * G.ccy[pi] - dx;
+                const T oy = si * G.ccx[pi] + ci * G.ccy[pi] - dy;
+                // tree j's bounding radius with a margin far above the rounding of the
+                // centre and circle coordinates (read here: nothing is live across a clip)
+                const T r = G.cr[pi] + (G.radius * T(1.0 + 1e-5) + T(1e-5));
+                pred = ox * ox + oy * oy <= r * r;
+                item = (uint32_t)q | ((uint32_t)pi << 16);
+            }
+            const uint64_t m = __ballot(pred);
+            if (pred) iq[in_ + __popcll(m & lanemask_lt())] = item;
+            in_ += __popcll(m);
+            u0 += WAVE;
+            if (in_ >= WAVE || (u0 >= items && in_ > 0)) {
+                const int c = in_ < WAVE ? in_ : WAVE;
+                wave_sync();
+                expand_items(tq, iq + in_ - c, c, pq, pn);
+                in_ -= c;
+                wave_sync();
             }
         }
-        wave_sync();
     }
 
     // Ballot-append a candidate to the wave's tree-pair queue; drain 64 entries when it
@@ -769,8 +817,8 @@ struct Evaluator {
     // within the cutoff to the queue, so the result does not depend on the mode.
     __device__ inline void run_grid(int list_mode) {
         const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-        uint32_t* tq = L.wq + wave * 2 * QCAP;
-        uint32_t* pq = tq + QCAP;
+        uint32_t* tq = L.wq + wave * L.wqs;
+        uint32_t* pq = tq + 2 * XQ;   // exact modes; the item queue is tq + XQ
         int tn = 0, pn = 0;
         const int N = L.N;
         if (list_mode == 2) {

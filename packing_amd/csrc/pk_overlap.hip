@@ -18,8 +18,12 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 // ---------------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------------
+// The float cost+gradient kernels are held to 128 VGPRs (four waves per SIMD): with four
+// 64-lane waves per instance that keeps four instances on every CU, so a batch of 1024
+// instances is resident at once instead of running a second, one-wave-per-SIMD round.
+// They fit without spilling; the other variants keep the compiler's own choice.
 template <typename T, int MODE, bool GRAD, bool CRYSTAL, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_overlap(const T* __restrict__ xyt,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && GRAD && !CRYSTAL) ? 4 : 1))) void k_overlap(const T* __restrict__ xyt,
                                                    const int* __restrict__ inst_off, int n_trees,
                                                    int n_ens, const GeomT<T>* __restrict__ geom,
                                                    const T* __restrict__ crystal, int npd,

@@ -122,7 +122,7 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     // vertex loop runs on full rows.
     {
         const int lane = threadIdx.x % WAVE;
-        uint32_t* bq = L.wq + (threadIdx.x / WAVE) * 2 * QCAP;
+        uint32_t* bq = L.wq + (threadIdx.x / WAVE) * L.wqs;
         int bn = 0;
         auto finish = [&](int i) {
             float gb[3], gh[3];
