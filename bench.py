@@ -6,7 +6,8 @@ theta ~ U(0, 2 pi), square h = [1.2 sqrt N, 0.6 sqrt N, 0.6 sqrt N] (fixed), cos
 pose-table overlap (256 x 256 x 128 float32 table built on the GPU from exact
 clipping) + boundary term, 500 L-BFGS iterations (history 8, max_step 0.1, no line
 search).  One step = relaxing the whole batch from the initial poses (reset + all
-iterations, replayed from a HIP graph of the chunked persistent launches).
+iterations, replayed from a HIP graph: each size bucket's chunked persistent launches back
+to back on a stream of its own, forked once and joined once).
 
 value = relaxations per second (instances x steps / time).  Printed as one JSON line.
 """
@@ -459,8 +460,7 @@ def main():
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph, stream=stream, capture_error_mode="relaxed"):
             b.reset()
-            for _ in range(n_chunks):
-                b.launch(chunk, stream=stream)
+            b.launch(chunk, stream=stream, chunks=n_chunks)
     for _ in range(args.warmup):
         graph.replay()
     torch.cuda.synchronize()

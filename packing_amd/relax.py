@@ -187,12 +187,16 @@ class RelaxBatch:
         N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), N.stream_handle(stream)),
                 "pk_relax_step")
 
-    def launch(self, iters: int, stream=None, **kw) -> None:
-        """Advance by up to ``iters`` iterations.  Without explicit order/n_launch, one
-        launch per size bucket: the largest on ``stream``, the others on side streams
-        forked from and joined back to it (a graph captures the fork and join)."""
+    def launch(self, iters: int, stream=None, chunks: int = 1, **kw) -> None:
+        """Advance by up to ``chunks`` x ``iters`` iterations.  Without explicit
+        order/n_launch, each size bucket runs its ``chunks`` launches back to back on its
+        own stream (the largest on ``stream``, the others on side streams forked from it),
+        joined back once at the end: a bucket that finishes a chunk early starts its next
+        one instead of waiting for the slowest bucket (a graph captures the fork and join).
+        Instances are independent, so the results are those of ``chunks`` calls."""
         if kw or len(self._buckets) <= 1:
-            self._launch_one(iters, stream, **kw)
+            for _ in range(chunks):
+                self._launch_one(iters, stream, **kw)
             return
         main = stream if stream is not None else torch.cuda.current_stream(self.device)
         fork = torch.cuda.Event()
@@ -203,8 +207,9 @@ class RelaxBatch:
             if k:
                 s.wait_event(fork)
             blk = int(self.opts.block) or self._block_for(nmax)
-            self._launch_one(iters, s, order=self.order[off:off + cnt], n_launch=cnt,
-                             n_tree_max=nmax, block=blk)
+            for _ in range(chunks):
+                self._launch_one(iters, s, order=self.order[off:off + cnt], n_launch=cnt,
+                                 n_tree_max=nmax, block=blk)
             if k:
                 e = torch.cuda.Event()
                 e.record(s)
@@ -232,9 +237,7 @@ class RelaxBatch:
     def run(self, iters_per_launch: int | None = None, stream=None) -> None:
         """Batched path: every instance in each launch, until max_iter."""
         k = iters_per_launch or self.opts.max_iter
-        n_launch = -(-self.opts.max_iter // k)
-        for _ in range(n_launch):
-            self.launch(k, stream=stream)
+        self.launch(k, stream=stream, chunks=-(-self.opts.max_iter // k))
 
     def run_noalloc(self, stream=None) -> None:
         """Relax the batch from its initial poses to the end in one call that allocates
@@ -259,9 +262,10 @@ class RelaxBatch:
             for _ in range(self.opts.max_iter):
                 self.launch(1, order=one, n_launch=1, n_tree_max=int(self.sizes[i]), stream=stream)
 
-    def capture_graph(self, iters_per_launch: int, stream=None):
-        """Capture one chunk launch (all instances, iters_per_launch iterations) as a graph;
-        replaying it advances the batch like run(iters_per_launch) does per launch."""
+    def capture_graph(self, iters_per_launch: int, stream=None, chunks: int = 1):
+        """Capture ``chunks`` chunk launches (all instances, iters_per_launch iterations
+        each; each size bucket's chunks back to back) as a graph; one replay advances the
+        batch like launch(iters_per_launch, chunks=chunks)."""
         s = stream or torch.cuda.Stream(device=self.device)
         g = torch.cuda.CUDAGraph()
         # warm the kernel attribute / lazy state outside the capture on the same stream
@@ -271,7 +275,7 @@ class RelaxBatch:
             N.check(N.lib().pk_relax_step(self._dg.handle, C.byref(p), N.stream_handle(s)), "warmup")
             torch.cuda.current_stream().synchronize()
             with torch.cuda.graph(g, stream=s, capture_error_mode="relaxed"):
-                self.launch(iters_per_launch, stream=s)
+                self.launch(iters_per_launch, stream=s, chunks=chunks)
         self._graph = g
         return g
 

@@ -104,6 +104,13 @@ def test_invariances():
         torch.cuda.synchronize()
     for i in range(len(sizes)):
         assert np.array_equal(g.xyt(i).cpu().numpy(), R[i]), ("graph", i)
+    # one replay of three chunks per size bucket, each bucket's chunks back to back
+    g3 = batch(geom, xs, hs, **kw)
+    graph3 = g3.capture_graph(10, chunks=3)
+    graph3.replay()
+    torch.cuda.synchronize()
+    for i in range(len(sizes)):
+        assert np.array_equal(g3.xyt(i).cpu().numpy(), R[i]), ("graph, chunks", i)
 
 
 @pytest.mark.parametrize("mode", ["area", "sep"])
