@@ -60,8 +60,12 @@ __device__ inline float h_gradient(const pk_relax_params& p, const unsigned long
 // Cost and gradient of the state x into g (all threads); returns the loss.  list_mode:
 // 1 rebuilds the neighbour list while evaluating, 2 evaluates from it (no cell grid).
 // Each tree's poses, sincos and boundary term are handled by the same thread, so the
-// staging needs no barrier between them.  HW: thread 0 writes the h gradient g[3N..3N+2]
-// (visible to its own wave on return); without, the caller forms it with h_gradient.
+// staging needs no barrier between them.  HW: g receives the whole gradient: the weighted
+// overlap and boundary terms of every tree, and (thread 0, visible to its own wave on
+// return) the h gradient g[3N..3N+2].  Without HW the caller forms the gradient itself: g
+// receives only the unweighted boundary gradient of the trees near the boundary (the
+// caller keeps every other entry of g at zero), the overlap gradient stays in L.g and the
+// h gradient is left to h_gradient.
 template <int MODE, int BLOCK, int LU = 2, bool HW = true>
 __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<float>& L, const RelaxArgs& A,
                                     const float* x, float* g, int N,
@@ -133,14 +137,14 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
             lb[3] += Acc<float>::local(gh[2]);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-                g[3 * i + k] = p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * gb[k];
+                g[3 * i + k] = HW ? p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * gb[k] : gb[k];
         };
         for (int base = threadIdx.x - lane; base < N; base += BLOCK) {
             const int i = base + lane;
             bool near = false;
             if (i < N) {
                 near = !boundary_inside(G, L.x[i], L.y[i], half, hx, hy);
-                if (!near) {
+                if (HW && !near) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
                         g[3 * i + k] = p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * 0.f;
@@ -843,13 +847,22 @@ __device__ __forceinline__ double wave_evaluate(const GeomT<float>& G, EvalLds<f
     __syncthreads();
     const double loss = relax_eval<MODE, WAVE * NWV, wave_lookups<RW>(), false>(G, L, A, xs, gs, N, bacc,
                                                                                 st_tp, st_pp, lm, cut_list);
-    // relax_eval ends with a barrier after the last write of gs and of the boundary sums;
-    // the lane holding element 3N + j forms the h gradient itself
+    // relax_eval ends with a barrier after the last write of gs and of the boundary sums.
+    // Each lane forms its own elements: the overlap gradient from the fixed-point sums (read
+    // in element order, no bank conflicts) plus the boundary gradient of near trees, left
+    // in gs (zero elsewhere; cleared again here), and the h gradient for element 3N + j.
     const int n3 = n - 3;
+    const pk_relax_params& p = A.p;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
         const int k = lane + WAVE * (row0 + r);
-        g[r] = k < n3 ? gs[k] : (k < n ? h_gradient(A.p, bacc, k - n3, x[r]) : 0.f);
+        if (k < n3) {
+            const float gb = gs[k];
+            g[r] = p.w_overlap * Acc<float>::get(L.g[k]) + p.w_boundary * gb;
+            gs[k] = 0.f;
+        } else {
+            g[r] = k < n ? h_gradient(p, bacc, k - n3, x[r]) : 0.f;
+        }
     }
     if (lm == 1) nb_valid = L.wcnt[8] <= L.nb_cap;
     return loss;
