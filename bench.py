@@ -270,6 +270,62 @@ def c1_plain_section(dev, iters=500):
     return out
 
 
+def c3_section(dev, cpu_sample=8):
+    """Differentiation + assignment configuration: the autograd overlap cost (forward with
+    the analytic gradient saved, backward) on 256 instances x 50 trees in FP32, and the
+    batched Hungarian on 512 int32 200 x 200 cost matrices with entries U[0, 10^6)
+    (assignments bit-exact: tested in tests/test_gpu_lap.py).  The CPU baseline of the
+    assignment is scipy's linear_sum_assignment on one core, on the first cpu_sample
+    matrices."""
+    import torch
+    from scipy.optimize import linear_sum_assignment
+    from packing_amd.autograd import overlap_cost
+    from packing_amd.geometry import make_poses
+    from packing_amd.lap_batch import solve_lap_batch
+    out = {}
+    x = torch.tensor(make_poses(np.random.default_rng(4), 256, 50, np.float32), device=dev)
+
+    def fwd_bwd():
+        xr = x.clone().requires_grad_(True)
+        overlap_cost(xr).sum().backward()
+        return xr.grad
+
+    for _ in range(3):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    out["autograd"] = {"workload": "256 instances x 50 trees, exact area, forward + backward, FP32",
+                       "ms_per_call": round(t * 1e3, 4), "instances_per_s": round(256 / t, 1)}
+    rng = np.random.default_rng(6)
+    C = rng.integers(0, 10 ** 6, size=(512, 200, 200)).astype(np.int32)
+    Cd = torch.tensor(C, device=dev)
+    s = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(s):
+        solve_lap_batch(Cd, stream=s)
+        s.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(3):
+            solve_lap_batch(Cd, stream=s)
+        e1.record(s)
+        s.synchronize()
+    ms = e0.elapsed_time(e1) / 3
+    t0 = time.perf_counter()
+    for b in range(cpu_sample):
+        linear_sum_assignment(C[b])
+    t_cpu = (time.perf_counter() - t0) / cpu_sample
+    out["assignment"] = {"workload": "512 int32 cost matrices 200 x 200, entries U[0, 10^6), Hungarian",
+                         "ms_per_batch": round(ms, 3), "problems_per_s": round(512 / (ms / 1e3), 1),
+                         "cpu_scipy": {"problems_per_s": round(1.0 / t_cpu, 1), "cores": 1,
+                                       "sample": f"{cpu_sample} matrices"}}
+    return out
+
+
 def c0_section(geom, dev, budget_s, iters=200):
     """The CPU configuration: 16 instances x 20 trees in FP64, exact overlap area +
     boundary, 200 L-BFGS iterations (history 8): the FP64 oracle on the host cores (OpenMP
@@ -522,10 +578,12 @@ def main():
                                   n_sat=args.serve_saturation)
         log(f"serving: {serving}")
 
-    c1 = c0 = None
+    c1 = c0 = c3 = None
     if not args.no_configs:
         c1 = c1_plain_section(dev)
         log(f"plain-method GPU configuration: {c1}")
+        c3 = c3_section(dev)
+        log(f"differentiation + assignment configuration: {c3}")
         if not args.no_cpu:
             c0 = c0_section(geom, dev, budget_s=5.0)
             log(f"plain-method CPU configuration: {c0}")
@@ -564,6 +622,7 @@ def main():
         "overlap_kernel": ovl,
         "plain_method_gpu_config": c1,
         "plain_method_cpu_config": c0,
+        "differentiation_assignment_config": c3,
         "final_loss_median": float(np.median(loss)),
         "table_build_s": round(t_lut, 4),
         "device": info,

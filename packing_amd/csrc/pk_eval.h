@@ -743,9 +743,9 @@ struct Evaluator {
     // holds every piece of tree j) meets no piece of j and is dropped; the others are
     // compacted into the item queue and expanded 64 at a time over the pieces of tree j.
     // Dropped pieces would clip to exactly zero, so the results are unchanged.
-    __device__ inline void expand(const uint32_t* tq, int tn, uint32_t* pq, int& pn) {
+    __device__ inline void expand(uint32_t* tq, int tn, uint32_t* pq, int& pn) {
         const int lane = threadIdx.x % WAVE;
-        uint32_t* iq = const_cast<uint32_t*>(tq) + XQ;
+        uint32_t* iq = tq + XQ;   // the item queue follows the tree-pair queue
         const int P = L.P;
         const float iP = 1.f / (float)P;
         const int items = tn * P;
