@@ -115,7 +115,12 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     ev.lut = A.lut;
     ev.cut_list = cut_list;
     ev.run(list_mode);
-    __syncthreads();
+    // the boundary pass below reads the fixed-point sums only with HW (without, it reads the
+    // poses, staged before the last barrier, and writes near trees' entries of g only)
+    if (HW)
+        __syncthreads();
+    else
+        wave_sync();   // the wave's last queue reads before the boundary pass reuses the queue
     st_tp += ev.n_tp;
     st_pp += ev.n_pp;
     const float half = x[3 * N] * 0.5f, hx = x[3 * N + 1], hy = x[3 * N + 2];
