@@ -3,7 +3,8 @@
 Requests (initial poses + box, in host memory) arrive over time.  The server keeps a
 device slot table (fixed-capacity slots of up to ``n_max`` trees: state, gradient, L-BFGS
 history and optimizer record in HBM) and admits a request into a free slot when its
-memory fits the HBM budget: its poses are copied host -> device then, inside its
+memory fits the HBM budget: its poses are staged in pinned host memory then and copied
+to the device with the next step (one copy per run of consecutive slots), inside its
 latency.  A step advances every occupied slot by ``chunk`` iterations of the fused
 engine (pk_relax_step), one launch per size class (the register rows R = ceil((3N + 3)
 / 64) of the one-wave kernel: R <= 3, 4-5, 6-8, 9-10, and larger), each with the kernel
@@ -11,8 +12,9 @@ for its own largest request, on forked streams that run concurrently.  Two steps
 kept in flight, so the GPU never waits for the host to retire and admit: a request
 admitted while step k runs joins step k + 1 or k + 2.  Finished slots are retired from
 the status copy of the step that finished them; results and the FP64 feasibility check
-(pk_feasible) come back to the host on a separate stream while the next step runs, and
-a request's latency ends when its result is in host memory.
+(pk_feasible) come back to the host on a separate stream while the next step runs, the
+host polling for them instead of waiting, and a request's latency ends when its result
+is in host memory.
 
 Each slot runs exactly the computation it would run alone (per-instance L-BFGS,
 fixed-point reductions), so a served result is bitwise the same as relaxing the request
@@ -135,6 +137,7 @@ class RelaxServer:
         self.bytes_in_use = 0
         self.per_slot = per
         self._steps: deque = deque()
+        self._retiring: deque = deque()    # retirements whose results are still on their way
         self._k = 0                        # steps launched (selects the staging buffers)
 
     # ---------------------------------------------------------------- admission
@@ -144,6 +147,8 @@ class RelaxServer:
         check_request(req, self.n_max)
 
     def _admit(self, req: Request, now: float) -> None:
+        """Take a free slot and stage the request's poses and box in the slot's pinned host
+        region; the host -> device copy is enqueued with the next step (_reset_pending)."""
         self.check_request(req)
         s = self.free.pop()
         n = req.xyt.shape[0]
@@ -151,8 +156,6 @@ class RelaxServer:
         xh = self.x_host[o:o + 3 * n + 3].numpy()
         xh[:3 * n] = np.asarray(req.xyt, dtype=np.float32).reshape(-1)
         xh[3 * n:] = np.asarray(req.h, dtype=np.float32).reshape(-1)
-        with torch.cuda.stream(self.stream):
-            self.x[o:o + 3 * n + 3].copy_(self.x_host[o:o + 3 * n + 3], non_blocking=True)
         req.slot = s
         req.t_admit = now
         self.busy[s] = req
@@ -160,8 +163,10 @@ class RelaxServer:
         self.bytes_in_use += self.per_slot
 
     def _reset_pending(self) -> None:
-        """Fresh optimizer records, status and tree counts for the slots admitted since the
-        last step (one index copy for all of them)."""
+        """The slots admitted since the last step: their staged poses copied to the device
+        (one copy per run of consecutive slots; a slot's whole region is copied, the part
+        beyond its 3N + 3 values is never read), fresh optimizer records, status and tree
+        counts (one index copy for all of them)."""
         if not self.pending:
             return
         k = len(self.pending)
@@ -169,8 +174,14 @@ class RelaxServer:
         a = buf.numpy()
         a[:k] = self.pending
         a[k:2 * k] = [self.busy[s].xyt.shape[0] for s in self.pending]
+        runs = np.sort(np.asarray(self.pending))
+        cut = np.flatnonzero(np.diff(runs) != 1) + 1
         self.pending = []
         with torch.cuda.stream(self.stream):
+            w = self.stride
+            for r in np.split(runs, cut):
+                lo, hi = int(r[0]) * w, (int(r[-1]) + 1) * w
+                self.x[lo:hi].copy_(self.x_host[lo:hi], non_blocking=True)
             d = buf[:2 * k].to(self.dev, non_blocking=True)
             idx, nt = d[:k], d[k:].to(torch.int32)
             self.state.view(self.S, -1).index_fill_(0, idx, 0)
@@ -250,9 +261,12 @@ class RelaxServer:
         self._steps.append(_Step(ev, sh, {s: self.busy[s] for s in slots}))
         self._k += 1
 
-    def _retire(self, step: _Step, t0: float, done_list: list) -> None:
-        """Retire the slots that the finished step left done: results and feasibility on the
-        retire stream (the next step keeps running), then back to the host."""
+    def _retire_begin(self, step: _Step) -> None:
+        """Retire the slots that the finished step left done: their results and the FP64
+        feasibility check are enqueued on the retire stream (the next step keeps running)
+        into pinned host buffers, with an event that _retire_end waits for; the host is not
+        blocked.  The slots stay allocated until then (a new request could otherwise
+        overwrite a result before it is copied out)."""
         st = step.status_h.numpy()
         bad = [s for s in step.slots if st[s] < 0]
         if bad:
@@ -265,19 +279,32 @@ class RelaxServer:
         rs = self.retire_stream
         rs.wait_event(step.event)
         sizes = [r.xyt.shape[0] for r in reqs]
+        k = len(fin)
+        # slot ids and instance offsets through pinned memory: the copies stay asynchronous
+        meta = torch.empty(2 * k + 1, dtype=torch.int64, pin_memory=True)
+        mv = meta.numpy()
+        mv[:k] = fin
+        mv[k:] = np.concatenate([[0], np.cumsum(sizes)])
+        host = torch.empty(3 * int(mv[-1]), dtype=torch.float32, pin_memory=True)
+        pairs_h = torch.empty(k, dtype=torch.int32, pin_memory=True)
+        loss_h = torch.empty(k, dtype=torch.float32, pin_memory=True)
         with torch.cuda.stream(rs):
             # done slots are never written again by a step, so reading them races with nothing
             flat = torch.cat([self.x[s * self.stride:s * self.stride + 3 * n] for s, n in zip(fin, sizes)])
-            off = torch.tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)).to(self.dev,
-                                                                                          non_blocking=True)
+            md = meta.to(self.dev, non_blocking=True)
+            off = md[k:].to(torch.int32)
             pairs = overlapping_pairs(flat.view(-1, 3), off, max(sizes), stream=rs, geometry=self.geometry)
-            idx = torch.tensor(fin, dtype=torch.int64).to(self.dev, non_blocking=True)
-            losses = self.loss[idx]
-            host = torch.empty(flat.numel(), dtype=torch.float32).pin_memory()
             host.copy_(flat, non_blocking=True)
-            pairs_h = pairs.to("cpu", non_blocking=True)
-            loss_h = losses.to("cpu", non_blocking=True)
-        rs.synchronize()
+            pairs_h.copy_(pairs, non_blocking=True)
+            loss_h.copy_(self.loss[md[:k]], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(rs)
+        self._retiring.append((ev, fin, reqs, sizes, host, pairs_h, loss_h, meta))
+
+    def _retire_end(self, t0: float, done_list: list) -> None:
+        """Hand the oldest retirement's results to its requests (its event has completed)
+        and free their slots."""
+        ev, fin, reqs, sizes, host, pairs_h, loss_h, _ = self._retiring.popleft()
         now = time.perf_counter() - t0
         k = 0
         for j, (s, r) in enumerate(zip(fin, reqs)):
@@ -316,9 +343,12 @@ class RelaxServer:
                 n_launch += 1
                 continue
             if self._steps and self._steps[0].event.query():
-                self._retire(self._steps.popleft(), t0, done)
+                self._retire_begin(self._steps.popleft())
                 continue
-            if self._steps:
+            if self._retiring and self._retiring[0][0].query():
+                self._retire_end(t0, done)
+                continue
+            if self._steps or self._retiring:
                 time.sleep(poll_s)
             elif k < len(reqs):
                 time.sleep(max(0.0, min(reqs[k].arrival - (time.perf_counter() - t0), 0.01)))
