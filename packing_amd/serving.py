@@ -89,6 +89,10 @@ class _Step:
 
 
 class RelaxServer:
+    """Continuous-batching server (see the module docstring).  chunk: iterations per step
+    (100 measured at least as good as 50 for p50/p99 at 2000 requests/s and better for
+    capacity; 25 and 200 worse)."""
+
     def __init__(self, opts: RelaxOptions, n_slots: int = 4096, n_max: int = 200,
                  hbm_budget_bytes: float = 64e9, chunk: int = 100, device="cuda", geometry=None,
                  in_flight: int = 2):
