@@ -125,7 +125,8 @@ constexpr int TP_CAP = 1024;   // block-wide tree-pair list capacity (periodic i
 constexpr int PP_CAP = 2048;   // block-wide piece-pair list capacity (periodic images)
 constexpr int QCAP = 160;      // per-wave queue capacity (entries), tree pairs and piece pairs
 constexpr int XQ = 128;        // exact modes: per-wave tree-pair, piece-item and piece-pair
-                               // queues of XQ entries each (each holds at most 127)
+                               // queues and the wave's share of the final tree pairs, XQ
+                               // entries each (each holds at most 127)
 constexpr int LUT_UMAX = 4;    // most table lookups per lane per drain (the queue holds
                                // 2 QCAP >= 64 LUT_UMAX + 63 entries)
 static_assert(2 * QCAP >= WAVE * LUT_UMAX + WAVE - 1, "table queue capacity");
@@ -157,18 +158,20 @@ struct EvalLds {
     int* next;     // [N]   grid: next tree in the same bucket
     uint32_t* cell;   // [N] grid: biased cell coordinates cx | cy << 16
     uint32_t* wq;  // [waves * wqs] per-wave queue space: table mode one queue of 2 QCAP
-                   // entries; exact modes tree pairs, piece items, piece pairs (XQ each)
+                   // entries; exact modes tree pairs, piece items, piece pairs and the
+                   // shared-out final tree pairs (XQ each)
     uint32_t* tp;  // [TP_CAP] periodic images: tree-pair entries i | j << 12 | cell << 24
     uint32_t* pp;  // [PP_CAP] periodic images: piece-pair entries tp index | pi << 16 | pj << 20
     uint32_t* nb;  // [nb_cap] neighbour list (relaxation): pairs a | b << 12 within the
                    // cutoff plus a skin at the last rebuild; wcnt[8] holds its length
-    int* wcnt;     // [16] per-wave ballot counts; [8] neighbour-list length
+    int* wcnt;     // [16] per-wave ballot counts; [8] neighbour-list length; [12 + w] wave w's
+                   // tree pairs left for the shared final expansion
     double* wsum;  // [MAX_WAVES] per-wave cost partials (double precision)
     int N, P, NV, H, nb_cap, wqs;
 };
 
 // Entries of queue space per wave: exact modes (NV > 0) three queues of XQ, table mode 2 QCAP.
-__host__ __device__ inline int wave_queue_entries(int NV) { return NV > 0 ? 3 * XQ : 2 * QCAP; }
+__host__ __device__ inline int wave_queue_entries(int NV) { return NV > 0 ? 4 * XQ : 2 * QCAP; }
 
 // Neighbour-list capacity for N trees (pairs).
 __host__ __device__ inline int nb_capacity(int N) { return 6 * N + 64; }
@@ -689,11 +692,11 @@ struct Evaluator {
         return gap > T(0) && gap * gap > mu * mu * (ddx * ddx + ddy * ddy);
     }
 
-    // Pieces pj of tree j against the items of iq (tree pair q | piece pi of tree i << 16)
-    // held by lanes below cnt: the item's poses and piece pi's circle are read once per run
-    // of pj (the same pj on every lane: broadcast reads); a run ends when the piece-pair
-    // queue holds 64 pairs, which are clipped before the lane state is read again (so that
-    // no lane state is live across the clip).
+    // Pieces pj of tree j against the items of iq (tree pair q of tq | piece pi of tree i
+    // << 16) held by lanes below cnt: the item's poses and piece pi's circle are read once
+    // per run of pj (the same pj on every lane: broadcast reads); a run ends when the
+    // piece-pair queue holds 64 pairs, which are clipped before the lane state is read again
+    // (so that no lane state is live across the clip).
     __device__ __attribute__((always_inline)) inline void expand_items(const uint32_t* tq, const uint32_t* iq,
                                                                         int cnt, uint32_t* pq, int& pn) {
         const int lane = threadIdx.x % WAVE;
@@ -720,8 +723,7 @@ struct Evaluator {
                     bool pred = act && ddx * ddx + ddy * ddy <= rr * rr;
                     if (pred) pred = !centre_axis_separated(i, j, pi, pj, dx, dy, ddx, ddy);
                     const uint64_t m = __ballot(pred);
-                    if (pred)
-                        pq[pn + __popcll(m & lanemask_lt())] = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
+                    if (pred) pq[pn + __popcll(m & lanemask_lt())] = e | ((uint32_t)pi << 24) | ((uint32_t)pj << 28);
                     pn += __popcll(m);
                     ++pj;
                     if (pn >= WAVE) break;
@@ -738,14 +740,14 @@ struct Evaluator {
     }
 
     // Expand the tn tree pairs of tq into the piece pairs whose bounding circles meet,
-    // clipping 64 at a time from the wave's piece-pair queue.  First one lane per (tree
-    // pair, piece pi of tree i): a piece whose circle misses tree j's bounding circle (which
-    // holds every piece of tree j) meets no piece of j and is dropped; the others are
-    // compacted into the item queue and expanded 64 at a time over the pieces of tree j.
-    // Dropped pieces would clip to exactly zero, so the results are unchanged.
-    __device__ inline void expand(uint32_t* tq, int tn, uint32_t* pq, int& pn) {
+    // clipping 64 at a time from this wave's piece-pair queue.  First one lane per (tree
+    // pair, piece pi of its first tree): a piece whose circle misses the second tree's
+    // bounding circle (which holds every piece of that tree) meets none of its pieces and is
+    // dropped; the others are compacted into the item queue iq and expanded 64 at a time
+    // over the pieces of the second tree.  Dropped pieces would clip to exactly zero, so
+    // the results are unchanged.
+    __device__ inline void expand(const uint32_t* tq, int tn, uint32_t* iq, uint32_t* pq, int& pn) {
         const int lane = threadIdx.x % WAVE;
-        uint32_t* iq = tq + XQ;   // the item queue follows the tree-pair queue
         const int P = L.P;
         const float iP = 1.f / (float)P;
         const int items = tn * P;
@@ -788,25 +790,59 @@ struct Evaluator {
 
     // Ballot-append a candidate to the wave's tree-pair queue; drain 64 entries when it
     // holds at least 64 (table lookups, or expansion into piece pairs).  Wave-uniform.
-    __device__ inline void push(bool pred, uint32_t entry, uint32_t* tq, int& tn, uint32_t* pq, int& pn) {
+    // Exact modes, last: expand what is left (the expansion's one call site, so the clip is
+    // inlined once); in four-wave workgroups the pairs left in all waves' queues are split
+    // evenly over the waves first (which wave found a pair depends on where its trees lie,
+    // and the busiest wave bounds the workgroup): each wave copies its share into its
+    // fourth queue region and expands it there.  That step has a workgroup barrier, so
+    // every wave makes the last call; the tree-pair queues and counts are used again only
+    // after the caller's barrier.
+    __device__ inline void push(bool pred, uint32_t entry, uint32_t* tq, int& tn, uint32_t* pq, int& pn,
+                                bool last = false) {
         const int lane = threadIdx.x % WAVE;
         const uint64_t m = __ballot(pred);
         if (m) {
             if (pred) tq[tn + __popcll(m & lanemask_lt())] = entry;
             tn += __popcll(m);
             wave_sync();
-            if (MODE == MODE_LUT) {
-                // the table queue uses both halves of the wave's queue space (2 QCAP entries)
-                if (tn >= LU * WAVE) {
-                    lut_pairs(tq + tn - LU * WAVE, LU * WAVE);
-                    tn -= LU * WAVE;
-                    wave_sync();
-                }
-            } else if (tn >= WAVE) {
-                expand(tq, tn, pq, pn);
-                tn = 0;
+        }
+        if (MODE == MODE_LUT) {
+            // the table queue uses both halves of the wave's queue space (2 QCAP entries)
+            if (tn >= LU * WAVE) {
+                lut_pairs(tq + tn - LU * WAVE, LU * WAVE);
+                tn -= LU * WAVE;
                 wave_sync();
             }
+        } else if (tn >= WAVE || last) {
+            const uint32_t* src = tq;
+            int cnt = tn;
+            if (NW == MAX_WAVES && last) {   // four-wave workgroups (the registers of the
+                                             // two-wave register kernels have no room for it)
+                if (lane == 0) L.wcnt[12 + threadIdx.x / WAVE] = tn;
+                __syncthreads();
+                int base[MAX_WAVES + 1];
+                base[0] = 0;
+#pragma unroll
+                for (int w = 0; w < MAX_WAVES; ++w)   // wave-uniform
+                    base[w + 1] = base[w] + (w < NW ? __builtin_amdgcn_readfirstlane(L.wcnt[12 + w]) : 0);
+                const int total = base[NW], share = (total + NW - 1) / NW;
+                const int lo = min(total, (int)(threadIdx.x / WAVE) * share);
+                cnt = min(total, lo + share) - lo;
+                uint32_t* gq = tq + 3 * XQ;
+                for (int k = lane; k < cnt; k += WAVE) {
+                    const int g = lo + k;
+                    int qoff = g;   // position of pair g in the queue space
+#pragma unroll
+                    for (int v = 1; v < MAX_WAVES; ++v)
+                        if (v < NW && g >= base[v]) qoff = v * L.wqs + (g - base[v]);
+                    gq[k] = L.wq[qoff];
+                }
+                wave_sync();
+                src = gq;
+            }
+            expand(src, cnt, tq + XQ, pq, pn);
+            tn = 0;
+            wave_sync();
         }
     }
 
@@ -907,7 +943,7 @@ struct Evaluator {
         if (MODE == MODE_LUT) {
             lut_pairs(tq, tn);
         } else {
-            expand(tq, tn, pq, pn);
+            push(false, 0u, tq, tn, pq, pn, true);
             if (lane < pn) clip_entry(pq[lane]);
             n_pp += pn;
         }

@@ -115,9 +115,12 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     ev.lut = A.lut;
     ev.cut_list = cut_list;
     ev.run(list_mode);
-    // the boundary pass below reads the fixed-point sums only with HW (without, it reads the
-    // poses, staged before the last barrier, and writes near trees' entries of g only)
-    if (HW)
+    // The boundary pass below reads the fixed-point sums only with HW (without, it reads the
+    // poses, staged before the last barrier, and writes near trees' entries of g only), and
+    // reuses this wave's queue space, which in the exact modes of four-wave workgroups the
+    // other waves read while they expand their share of all waves' tree pairs (so exact
+    // modes keep the barrier); in table mode only this wave does.
+    if (HW || MODE != MODE_LUT)
         __syncthreads();
     else
         wave_sync();   // the wave's last queue reads before the boundary pass reuses the queue
