@@ -51,7 +51,9 @@ int pk_device_info(int* max_lds_per_block, int* n_cu, int* clock_khz);
  * core/pack_cuda.py:1005-1030 (pieces from _build_convex_piece_arrays, :787-849; tree
  * vertices from core/kaggle_support.py:351-354) with a handle, so several polygons can
  * coexist.  1 <= n_pieces <= 16, 3 <= vertices per piece <= max_verts <= 4,
- * n_outline <= 64.
+ * n_outline <= 64, every piece vertex within 8 of the origin (pair terms are summed in
+ * int64 fixed point at 2^-40, each term below 2000 in magnitude; the reference's tree
+ * has radius ~0.8).  PK_EINVAL otherwise.
  */
 int pk_geometry_create(const double* piece_xy, const int32_t* piece_nverts, int n_pieces,
                        int max_verts, const double* outline_xy, int n_outline, double radius,
@@ -94,10 +96,14 @@ int pk_boundary(pk_geometry_t g, int dtype, const void* xyt, const int32_t* inst
  * mirror = 0 is a full-range table for polygons without mirror symmetry.
  * quad = 1 applies max(0, v)^2 per pair (apply_quadratic_transform).  Limits: nx ny and
  * nt + 1 below 2^24, nx ny (nt + 1) below 2^32 (the reference's default 900^3 fits); a
- * mirrored table needs t_min = -t_max.  The handle keeps its own corner-quad copy of
+ * mirrored table needs t_min = -t_max.  Each lookup's value and gradient terms must stay
+ * below 2000 in magnitude (fixed-point pair sums): the call bounds them from the table's
+ * largest |value| and node steps and returns PK_ERANGE beyond (also for NaN or infinite
+ * values); overlap-area and separation tables of the tree are below 10.  The handle keeps its own corner-quad copy of
  * vals (4 (nt + 1) / nt times the table's memory, on the table's device): entry
  * (ix, iy, it) holds the four (x, y) corners of a cell at theta node min(it, nt - 1), so
- * a lookup reads two adjacent 16-byte entries instead of four scattered pairs.  The copy is taken here (the call synchronizes that device), so writes to vals
+ * a lookup reads two adjacent 16-byte entries instead of four scattered pairs.  The copy
+ * is taken here (the call synchronizes that device), so writes to vals
  * after pk_lut_create are not seen; create a new handle instead.
  */
 int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, double x_max,

@@ -74,6 +74,10 @@ struct Geometry {
 // which lanes, waves or workgroup shapes deliver them.  Doubles use double atomics.
 constexpr double FIX_SCALE = 1099511627776.0;   // 2^40
 constexpr double FIX_INV = 1.0 / 1099511627776.0;
+// Largest |term| that local() converts: pair terms stay below it because the geometry
+// handle keeps every piece vertex within 8 of the tree origin and the table handle
+// bounds the table's values and slopes (both checked when the handle is made).
+constexpr float FIX_TERM_MAX = 2000.f;
 
 template <typename T>
 struct Acc;
@@ -81,10 +85,23 @@ struct Acc;
 template <>
 struct Acc<float> {
     using type = unsigned long long;
+    // round(v 2^40), ties to even, for |v| < 2^11: v 2^40 is exact in float and in double;
+    // adding 1.5 2^52 leaves the nearest integer n in the low mantissa bits, so the bit
+    // pattern is 0x4338000000000000 + n (two's complement), and that constant's low word
+    // is zero.  Four instructions instead of the nine of a general double -> int64
+    // conversion, with the same result in that range.
     __device__ static inline type local(float v) {
+        const double d = (double)(v * 0x1p40f) + 0x1.8p52;
+        const uint32_t hi = (uint32_t)__double2hiint(d) - 0x43380000u;
+        return ((unsigned long long)hi << 32) | (uint32_t)__double2loint(d);
+    }
+    // the same rounding over the whole int64 range (boundary terms, which grow with the
+    // distance of a tree outside its box)
+    __device__ static inline type wide(float v) {
         return (unsigned long long)__double2ll_rn((double)v * FIX_SCALE);
     }
     __device__ static inline void add(type* p, float v) { atomicAdd(p, local(v)); }
+    __device__ static inline void add_wide(type* p, float v) { atomicAdd(p, wide(v)); }
     __device__ static inline void add_raw(type* p, type v) { atomicAdd(p, v); }
     __device__ static inline float get(type v) { return (float)((double)(long long)v * FIX_INV); }
     __device__ static inline type zero() { return 0ull; }
@@ -95,7 +112,9 @@ template <>
 struct Acc<double> {
     using type = double;
     __device__ static inline type local(double v) { return v; }
+    __device__ static inline type wide(double v) { return v; }
     __device__ static inline void add(type* p, double v) { atomicAdd(p, v); }
+    __device__ static inline void add_wide(type* p, double v) { atomicAdd(p, v); }
     __device__ static inline void add_raw(type* p, type v) { atomicAdd(p, v); }
     __device__ static inline double get(type v) { return v; }
     __device__ static inline type zero() { return 0.0; }

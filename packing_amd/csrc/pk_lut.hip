@@ -20,8 +20,9 @@ struct Lut {
 // clamped to the last node, so the eight corners of a cell are two adjacent entries, also
 // at the upper theta end.
 __global__ __launch_bounds__(256) void k_lut_corners(const float* __restrict__ vals, int nx, int ny, int nt,
-                                                     float4* __restrict__ quad) {
+                                                     float4* __restrict__ quad, uint32_t* __restrict__ bound) {
     const int ntc = nt + 1;
+    uint32_t m[4] = {0u, 0u, 0u, 0u};   // max |v|, |step| along x, y, theta (float bits)
     const long long total = (long long)nx * ny * ntc;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
@@ -35,6 +36,20 @@ __global__ __launch_bounds__(256) void k_lut_corners(const float* __restrict__ v
         e.z = vals[((long long)ix1 * ny + iy) * nt + it];
         e.w = vals[((long long)ix1 * ny + iy1) * nt + it];
         quad[q] = e;
+        if (q % ntc < nt) {   // each node (ix, iy, it) once: |v| and its steps to the next node
+            const float vn = it + 1 < nt ? vals[((long long)ix * ny + iy) * nt + it + 1] : e.x;
+            m[0] = max(m[0], __float_as_uint(fabsf(e.x)));
+            m[1] = max(m[1], __float_as_uint(fabsf(e.z - e.x)));
+            m[2] = max(m[2], __float_as_uint(fabsf(e.y - e.x)));
+            m[3] = max(m[3], __float_as_uint(fabsf(vn - e.x)));
+        }
+    }
+    // non-negative floats order as their bit patterns, and NaN above infinity
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        uint32_t v = m[k];
+        for (int o = WAVE / 2; o > 0; o /= 2) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+        if (threadIdx.x % WAVE == 0 && v) atomicMax(&bound[k], v);
     }
 }
 
@@ -165,19 +180,46 @@ int pk_lut_create(const float* vals, int nx, int ny, int nt, double x_min, doubl
     PK_CHECK_HIP(hipGetDevice(&prev));
     PK_CHECK_HIP(hipSetDevice(attr.device));
     float4* cq = nullptr;
+    uint32_t* db = nullptr;
+    uint32_t hb[4] = {0u, 0u, 0u, 0u};
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMalloc((void**)&cq, sizeof(float4) * (size_t)nodes);
+    if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof(hb));
+    if (e == hipSuccess) e = hipMemcpy(db, hb, sizeof(hb), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         const int blocks = (int)std::min<long long>((nodes + 255) / 256, 65536);
-        hipLaunchKernelGGL(k_lut_corners, dim3(blocks), dim3(256), 0, 0, vals, nx, ny, nt, cq);
+        hipLaunchKernelGGL(k_lut_corners, dim3(blocks), dim3(256), 0, 0, vals, nx, ny, nt, cq, db);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(hb, db, sizeof(hb), hipMemcpyDeviceToHost);
     }
+    if (db) (void)hipFree(db);
     (void)hipSetDevice(prev);
     if (e != hipSuccess) {
         if (cq) (void)hipFree(cq);
         set_error(std::string("pk_lut_create: ") + hipGetErrorString(e));
         return PK_EHIP;
+    }
+    // Pair terms are summed in fixed point with |term| < FIX_TERM_MAX.  A lookup's value
+    // is a convex combination of nodes and its slopes of node steps, so with |v| <= B and
+    // steps <= Dx, Dy, Dt the terms are at most: value B (B^2 squared), centre gradient
+    // Dx sx + Dy sy, the first tree's angle gradient Dx sx Ym + Dy sy Xm + Dt st (Xm, Ym
+    // the largest |x|, |y| in range), the second's Dt st; squaring multiplies slopes by 2B.
+    {
+        float f[4];
+        std::memcpy(f, hb, sizeof(f));
+        const double B = f[0], sx = (nx - 1) / (x_max - x_min), sy = (ny - 1) / (y_max - y_min),
+                     st = (nt - 1) / (t_max - t_min);
+        const double k = quad ? 2.0 * B : 1.0;
+        const double gx = k * f[1] * sx, gy = k * f[2] * sy, gt = k * f[3] * st;
+        const double Xm = std::max(std::fabs(x_min), std::fabs(x_max)), Ym = std::max(std::fabs(y_min), std::fabs(y_max));
+        const double worst = std::max(std::max(quad ? B * B : B, gx + gy), std::max(gx * Ym + gy * Xm + gt, gt));
+        if (!(worst <= (double)FIX_TERM_MAX)) {   // also NaN or infinite values
+            (void)hipFree(cq);
+            set_error("pk_lut_create: table values or slopes too large: a lookup term could reach " +
+                      std::to_string(worst) + " (the fixed-point pair sums take |term| <= 2000)");
+            return PK_ERANGE;
+        }
     }
     Lut* L = new Lut();
     L->corners = cq;

@@ -107,10 +107,10 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(const T* __restrict__ xyt,
         sincos_t(t, &sn, &cs);
         T g[3], gh[3];
         const T c = boundary_tree<T>(G, x, y, cs, sn, half, hx, hy, g, gh);
-        Acc<T>::add(&acc[0], c);
-        Acc<T>::add(&acc[1], gh[0]);
-        Acc<T>::add(&acc[2], gh[1]);
-        Acc<T>::add(&acc[3], gh[2]);
+        Acc<T>::add_wide(&acc[0], c);
+        Acc<T>::add_wide(&acc[1], gh[0]);
+        Acc<T>::add_wide(&acc[2], gh[1]);
+        Acc<T>::add_wide(&acc[3], gh[2]);
         if (out_grads) {
             out_grads[3 * (off + i)] = g[0];
             out_grads[3 * (off + i) + 1] = g[1];
@@ -223,6 +223,16 @@ int pk_geometry_create(const double* piece_xy, const int32_t* piece_nverts, int 
     PK_REQUIRE(max_verts >= 3 && max_verts <= MAXV, "pk_geometry_create: 3 <= max_verts <= 4");
     PK_REQUIRE(n_outline >= 3 && n_outline <= MAXO, "pk_geometry_create: 3 <= n_outline <= 64");
     PK_REQUIRE(radius > 0, "pk_geometry_create: radius must be positive");
+    // Pair terms are summed in fixed point with |term| < FIX_TERM_MAX.  With every vertex
+    // within R of the origin a piece pair contributes at most pi R^2 of area, 2 pi R of
+    // area per unit of translation and 2 pi R^2 per radian, or in separation mode a
+    // squared depth (2R)^2 with derivatives up to 20 R^2: R <= 8 keeps all below 1300.
+    for (int p = 0; p < n_pieces; ++p)
+        for (int v = 0; v < piece_nverts[p] && v < max_verts; ++v) {
+            const double x = piece_xy[(p * max_verts + v) * 2], y = piece_xy[(p * max_verts + v) * 2 + 1];
+            PK_REQUIRE(std::sqrt(x * x + y * y) <= 8.0,
+                       "pk_geometry_create: piece vertices must lie within 8 of the tree origin");
+        }
     Geometry* g = new Geometry();
     std::memset(&g->hf, 0, sizeof(g->hf));
     std::memset(&g->hd, 0, sizeof(g->hd));

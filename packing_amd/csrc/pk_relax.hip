@@ -139,10 +139,10 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
         auto finish = [&](int i) {
             float gb[3], gh[3];
             const float cb = boundary_tree<float, false>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
-            lb[0] += Acc<float>::local(cb);
-            lb[1] += Acc<float>::local(gh[0]);
-            lb[2] += Acc<float>::local(gh[1]);
-            lb[3] += Acc<float>::local(gh[2]);
+            lb[0] += Acc<float>::wide(cb);
+            lb[1] += Acc<float>::wide(gh[0]);
+            lb[2] += Acc<float>::wide(gh[1]);
+            lb[3] += Acc<float>::wide(gh[2]);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 g[3 * i + k] = HW ? p.w_overlap * Acc<float>::get(L.g[3 * i + k]) + p.w_boundary * gb[k] : gb[k];

@@ -51,3 +51,16 @@ def test_lut_create_rejects_asymmetric_mirror_table():
     rc = L.pk_lut_create(C.c_void_p(16), 8, 8, 8, -0.1, 1.0, -1.0, 1.0, -3.0, 3.1, 0, 1, C.byref(h))
     assert rc != 0
     assert b"symmetric" in L.pk_last_error()
+
+
+def test_geometry_create_rejects_far_vertices():
+    """Pair terms are summed in fixed point; the bound on them needs every piece vertex
+    within 8 of the tree origin, checked before any device work (no GPU needed)."""
+    import ctypes as C
+    L = _native.lib()
+    h = C.c_void_p()
+    sq = (C.c_double * 8)(*[9.0, 0.0, 10.0, 0.0, 10.0, 1.0, 9.0, 1.0])
+    nv = (C.c_int32 * 1)(4)
+    rc = L.pk_geometry_create(sq, nv, 1, 4, sq, 4, 10.1, C.byref(h))
+    assert rc != 0
+    assert b"within 8" in L.pk_last_error()

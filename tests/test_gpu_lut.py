@@ -148,3 +148,29 @@ def test_reference_default_table_size(oracle_mod, gname):
     assert checked >= 20
     del cc, L, vals
     torch.cuda.empty_cache()
+
+
+def test_table_term_bound():
+    """Lookup terms are summed in fixed point, each below 2000 in magnitude: a table whose
+    values or node steps could exceed that (or that holds NaN / inf) is refused when its
+    handle is made; the exact tables are far inside the bound."""
+    from packing_amd.pack_cuda_lut import LookupTable
+    X = np.linspace(-1.5, 1.5, 16)
+    Y = np.linspace(-1.5, 1.5, 16)
+    T = np.linspace(-np.pi, np.pi, 8)
+    base = np.random.default_rng(0).uniform(0, 0.3, size=(16, 16, 8)).astype(np.float32)
+    LookupTable(X, Y, T, base.copy(), mirror_x=False).handle("cuda")
+    big = base.copy()
+    big[3, 4, 5] = 2500.0                               # a value above the bound
+    steep = base.copy()
+    steep[7, 7, 3] = 150.0                              # |value| fine, slope x range is not
+    nan = base.copy()
+    nan[1, 2, 3] = np.nan
+    for vals, what in ((big, "value"), (steep, "slope"), (nan, "nan")):
+        with pytest.raises(ValueError, match="too large"):
+            LookupTable(X, Y, T, vals, mirror_x=False).handle("cuda")
+    # squared tables square the value bound: 40^2 = 1600 passes, 50^2 does not
+    flat = np.full((16, 16, 8), 40.0, np.float32)
+    LookupTable(X, Y, T, flat, apply_quadratic_transform=True, mirror_x=False).handle("cuda")
+    with pytest.raises(ValueError, match="too large"):
+        LookupTable(X, Y, T, flat + 10.0, apply_quadratic_transform=True, mirror_x=False).handle("cuda")
