@@ -1125,8 +1125,9 @@ __device__ inline bool boundary_inside(const GeomT<T>& G, T x, T y, T half, T hx
 // Boundary term of one tree (core/pack_cuda.py:550-642): max over outline vertices of
 // the squared distance outside the square h = [size, ox, oy]; gradients w.r.t. the pose
 // and h at the arg-max vertex (strict >, first vertex on ties, vertex 0 and zero if
-// inside).  The outline is padded to MAXO with its last vertex, so whole blocks of 8 are
-// read (one LDS round trip each); the arg-max vertex is carried, not read again.  CHECK:
+// inside).  G's outline holds vertex 0 and the vertices on the outline's convex hull, where
+// that maximum lies (pk_geometry_create), padded with its last vertex, so whole blocks of
+// 8 are read (one LDS round trip each); the arg-max vertex's index is carried.  CHECK:
 // test the bounding circle first (callers that already know the tree is near the
 // boundary skip it).
 template <typename T, bool CHECK = true>
@@ -1138,7 +1139,7 @@ __device__ inline T boundary_tree(const GeomT<T>& G, T x, T y, T c, T s, T half,
         return T(0);
     }
     T best = T(0);
-    T lx = G.ox[0], ly = G.oy[0];
+    int bi = 0;
     const int no = G.n_outline;
     for (int v0 = 0; v0 < no; v0 += 8) {
         T lxs[8], lys[8];
@@ -1155,11 +1156,11 @@ __device__ inline T boundary_tree(const GeomT<T>& G, T x, T y, T c, T s, T half,
             const T d2 = dx * dx + dy * dy;
             if (d2 > best) {
                 best = d2;
-                lx = lxs[u];
-                ly = lys[u];
+                bi = v0 + u;
             }
         }
     }
+    const T lx = G.ox[bi], ly = G.oy[bi];
     const T vx = c * lx - s * ly + x - hx, vy = s * lx + c * ly + y - hy;
     const T ax = fabs(vx), ay = fabs(vy);
     const T dx = ax > half ? ax - half : T(0), dy = ay > half ? ay - half : T(0);

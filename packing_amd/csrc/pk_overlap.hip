@@ -206,6 +206,42 @@ static int dispatch_overlap(const Geometry* g, const void* xyt, const int* inst_
 
 using namespace pk;
 
+// Indices (ascending) of the outline vertices that are vertex 0 or lie on the boundary of
+// the outline's convex hull (within 1e-9 of a hull edge's line); returns their count.
+static int outline_hull_subset(const double* xy, int n, int* keep) {
+    int idx[MAXO];
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx, idx + n, [&](int a, int b) {
+        return xy[2 * a] < xy[2 * b] || (xy[2 * a] == xy[2 * b] && xy[2 * a + 1] < xy[2 * b + 1]);
+    });
+    auto cross = [&](int o, int a, int b) {
+        return (xy[2 * a] - xy[2 * o]) * (xy[2 * b + 1] - xy[2 * o + 1]) -
+               (xy[2 * a + 1] - xy[2 * o + 1]) * (xy[2 * b] - xy[2 * o]);
+    };
+    int hull[2 * MAXO], k = 0;   // Andrew's monotone chain, counter-clockwise, strict turns
+    for (int i = 0; i < n; ++i) {
+        while (k >= 2 && cross(hull[k - 2], hull[k - 1], idx[i]) <= 0) --k;
+        hull[k++] = idx[i];
+    }
+    for (int i = n - 2, lo = k + 1; i >= 0; --i) {
+        while (k >= lo && cross(hull[k - 2], hull[k - 1], idx[i]) <= 0) --k;
+        hull[k++] = idx[i];
+    }
+    const int h = k - 1;   // the last point repeats the first
+    int c = 0;
+    for (int v = 0; v < n; ++v) {
+        bool on = v == 0 || h < 3;
+        for (int e = 0; e < h && !on; ++e) {
+            const int a = hull[e], b = hull[e + 1];
+            const double ex = xy[2 * b] - xy[2 * a], ey = xy[2 * b + 1] - xy[2 * a + 1];
+            const double dist = cross(a, b, v) / std::sqrt(ex * ex + ey * ey);
+            on = dist <= 1e-9;   // inside the hull every edge line has the point on its left
+        }
+        if (on) keep[c++] = v;
+    }
+    return c;
+}
+
 // ---------------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------------
@@ -291,10 +327,19 @@ int pk_geometry_create(const double* piece_xy, const int32_t* piece_nverts, int 
         g->hd.pv[p] = g->hf.pv[p] = word;
     }
     g->hd.NU = g->hf.NU = nu;
-    // the outline padded to MAXO by repeating its last vertex (boundary_tree reads whole
-    // blocks of 8; a repeat never wins its strict arg-max)
+    // The boundary term is the largest squared distance of an outline vertex outside the
+    // square: a convex function of the vertex, whose maximum over the outline is attained
+    // on the boundary of its convex hull, and (for a positive maximum) never first at a
+    // vertex strictly inside the hull.  The device keeps vertex 0 (the arg-max when no
+    // vertex is outside) and every vertex on the hull's boundary, collinear ones included,
+    // in their original order, so the strict first-wins arg-max selects the same vertex:
+    // 5 of the tree's 15 vertices.  Padded to MAXO by repeating the last kept vertex
+    // (boundary_tree reads whole blocks of 8; a repeat never wins its strict arg-max).
+    int keep[MAXO];
+    const int n_keep = outline_hull_subset(outline_xy, n_outline, keep);
+    g->hd.n_outline = g->hf.n_outline = n_keep;
     for (int v = 0; v < MAXO; ++v) {
-        const int w = v < n_outline ? v : n_outline - 1;
+        const int w = keep[v < n_keep ? v : n_keep - 1];
         g->hd.ox[v] = outline_xy[2 * w];
         g->hd.oy[v] = outline_xy[2 * w + 1];
         g->hf.ox[v] = (float)outline_xy[2 * w];
