@@ -210,15 +210,45 @@ __device__ inline double xchg_d(double v, int bit) {
     return __hiloint2double(xchg_i(__double2hiint(v), bit), xchg_i(__double2loint(v), bit));
 }
 
-// (send, keep) = up ? (lo, hi) : (hi, lo), as a bitwise blend: a select between two
-// elements of a register array can be rewritten by the compiler into a dynamically
-// indexed load, which moves the whole array to scratch memory.
-__device__ inline void blend_pair(double lo, double hi, bool up, double& send, double& keep) {
-    const unsigned long long m = up ? ~0ull : 0ull;
-    const unsigned long long a = __double_as_longlong(lo), b = __double_as_longlong(hi);
-    send = __longlong_as_double((a & m) | (b & ~m));
-    keep = __longlong_as_double((b & m) | (a & ~m));
+// One stage of the many-value butterfly below for lane bit `bit` on the pair (lo, hi):
+// returns a (this lane's) and b (the partner's) such that a lane with the bit clear
+// combines lo with its partner's lo and a lane with the bit set hi with its partner's hi.
+// Across rows (bits 16 and 32) one v_permlane16/32_swap of the pair delivers both:
+// it swaps the odd 16- (32-) lane rows of lo with the even rows of hi, leaving
+// a = (lo_l, hi_{l-b}) and b = (lo_{l+b}, hi_l) on the clear and set lanes, the operands
+// of the lane-exchange form in swapped order for the set lanes (sum and max are
+// symmetric).  Within a row: a bitwise blend (a select between two elements of a register
+// array can be rewritten by the compiler into a dynamically indexed load, which moves the
+// whole array to scratch memory) and a DPP move.
+__device__ inline void stage_i(int lo, int hi, int bit, bool up, int& a, int& b) {
+    if (bit == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+        a = (int)p[0];
+        b = (int)p[1];
+    } else if (bit == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+        a = (int)p[0];
+        b = (int)p[1];
+    } else {
+        const int m = up ? ~0 : 0;
+        a = (hi & m) | (lo & ~m);
+        b = xchg_i((lo & m) | (hi & ~m), bit);
+    }
 }
+__device__ inline void stage_f(float lo, float hi, int bit, bool up, float& a, float& b) {
+    int ai, bi;
+    stage_i(__float_as_int(lo), __float_as_int(hi), bit, up, ai, bi);
+    a = __int_as_float(ai);
+    b = __int_as_float(bi);
+}
+__device__ inline void stage_d(double lo, double hi, int bit, bool up, double& a, double& b) {
+    int al, bl, ah, bh;
+    stage_i(__double2loint(lo), __double2loint(hi), bit, up, al, bl);
+    stage_i(__double2hiint(lo), __double2hiint(hi), bit, up, ah, bh);
+    a = __hiloint2double(ah, al);
+    b = __hiloint2double(bh, bl);
+}
+
 
 // Sums of K values (K a power of two, 2..64) over the 64 lanes of a wave in one
 // butterfly.  At stage s the lanes l and its partner for bit b = 32 >> s (xchg_i: same
@@ -240,14 +270,18 @@ __device__ inline double wave_sum_many(double (&v)[K]) {
         const bool up = (lane & bit) != 0;
 #pragma unroll
         for (int j = 0; j < h; ++j) {
-            double send, keep;
-            blend_pair(v[j], v[h + j], up, send, keep);
-            v[j] = keep + xchg_d(send, bit);
+            double a, b;
+            stage_d(v[j], v[h + j], bit, up, a, b);
+            v[j] = a + b;
         }
     }
     double r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r += xchg_d(r, 32 >> s);
+    for (int s = LK; s < 6; ++s) {
+        double a, b;
+        stage_d(r, r, 32 >> s, (lane & (32 >> s)) != 0, a, b);
+        r = a + b;
+    }
     return r;
 }
 // The same for float partial sums, added in double.
@@ -259,12 +293,6 @@ __device__ inline double wave_sum_many(float (&v)[K]) {
     return wave_sum_many<K>(dv);
 }
 // The same butterfly in float (one exchange per value and stage).
-__device__ inline void blend_pair_f(float lo, float hi, bool up, float& send, float& keep) {
-    const unsigned m = up ? ~0u : 0u;
-    const unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
-    send = __uint_as_float((a & m) | (b & ~m));
-    keep = __uint_as_float((b & m) | (a & ~m));
-}
 template <int K>
 __device__ inline float wave_sum_many_f(float (&v)[K]) {
     constexpr int LK = ilog2c(K);
@@ -277,14 +305,18 @@ __device__ inline float wave_sum_many_f(float (&v)[K]) {
         const bool up = (lane & bit) != 0;
 #pragma unroll
         for (int j = 0; j < h; ++j) {
-            float send, keep;
-            blend_pair_f(v[j], v[h + j], up, send, keep);
-            v[j] = keep + xchg_f(send, bit);
+            float a, b;
+            stage_f(v[j], v[h + j], bit, up, a, b);
+            v[j] = a + b;
         }
     }
     float r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r += xchg_f(r, 32 >> s);
+    for (int s = LK; s < 6; ++s) {
+        float a, b;
+        stage_f(r, r, 32 >> s, (lane & (32 >> s)) != 0, a, b);
+        r = a + b;
+    }
     return r;
 }
 // The same layout with max instead of sum (exact in any order).
@@ -300,14 +332,18 @@ __device__ inline double wave_max_many(double (&v)[K]) {
         const bool up = (lane & bit) != 0;
 #pragma unroll
         for (int j = 0; j < h; ++j) {
-            double send, keep;
-            blend_pair(v[j], v[h + j], up, send, keep);
-            v[j] = fmax(keep, xchg_d(send, bit));
+            double a, b;
+            stage_d(v[j], v[h + j], bit, up, a, b);
+            v[j] = fmax(a, b);
         }
     }
     double r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r = fmax(r, xchg_d(r, 32 >> s));
+    for (int s = LK; s < 6; ++s) {
+        double a, b;
+        stage_d(r, r, 32 >> s, (lane & (32 >> s)) != 0, a, b);
+        r = fmax(a, b);
+    }
     return r;
 }
 // The same in float (max is exact, so the result equals the double version's).
@@ -323,14 +359,18 @@ __device__ inline float wave_max_many_f(float (&v)[K]) {
         const bool up = (lane & bit) != 0;
 #pragma unroll
         for (int j = 0; j < h; ++j) {
-            float send, keep;
-            blend_pair_f(v[j], v[h + j], up, send, keep);
-            v[j] = fmaxf(keep, xchg_f(send, bit));
+            float a, b;
+            stage_f(v[j], v[h + j], bit, up, a, b);
+            v[j] = fmaxf(a, b);
         }
     }
     float r = v[0];
 #pragma unroll
-    for (int s = LK; s < 6; ++s) r = fmaxf(r, xchg_f(r, 32 >> s));
+    for (int s = LK; s < 6; ++s) {
+        float a, b;
+        stage_f(r, r, 32 >> s, (lane & (32 >> s)) != 0, a, b);
+        r = fmaxf(a, b);
+    }
     return r;
 }
 // Lane holding value j of a K-value butterfly result.
