@@ -141,17 +141,21 @@ __host__ __device__ inline int grid_buckets(int N) {
     return h;
 }
 
+// A tree's centre and the cosine and sine of its angle: one 16-byte LDS read (two in
+// double) gives all four, a pair's centre difference two 8-byte reads.
+template <typename T>
+struct alignas(16) Pose {
+    T x, y, c, s;
+};
+
 // LDS view for one instance.
 template <typename T>
 struct EvalLds {
     using A = typename Acc<T>::type;
     A* g;          // [N*3] gradient accumulators
     A* cost;       // [1]   cost accumulator
-    T* x;          // [N]   poses
-    T* y;
-    T* t;
-    T* c;          // [N]   cos, sin of theta
-    T* s;
+    Pose<T>* p;    // [N]   centres, cos and sin of the angles
+    T* t;          // [N]   angles
     T* v;          // [N*NV*2] distinct piece vertices rotated about the tree centre,
                    // (x, y) interleaved: vertex u of tree i at v[2 (i NV + u)]
     int* head;     // [H]   grid: first tree of each bucket (-1: empty)
@@ -187,7 +191,7 @@ __host__ __device__ inline size_t eval_lds_bytes(int N, int NV, bool periodic = 
     size_t b = 0;
     b += sizeof(typename Acc<T>::type) * (3 * (size_t)N + 1);
     b = align_up(b, 16);
-    b += sizeof(T) * 5 * (size_t)N;
+    b += (sizeof(Pose<T>) + sizeof(T)) * (size_t)N;
     b = align_up(b, 16);
     b += sizeof(T) * 2 * (size_t)N * NV;
     b = align_up(b, 16);
@@ -213,9 +217,9 @@ __device__ inline EvalLds<T> carve_eval_lds(char* base, int N, int NV, bool peri
     L.cost = L.g + 3 * N;
     o += sizeof(typename Acc<T>::type) * (3 * (size_t)N + 1);
     o = align_up(o, 16);
-    T* f = reinterpret_cast<T*>(base + o);
-    L.x = f; L.y = f + N; L.t = f + 2 * N; L.c = f + 3 * N; L.s = f + 4 * N;
-    o += sizeof(T) * 5 * (size_t)N;
+    L.p = reinterpret_cast<Pose<T>*>(base + o);
+    L.t = reinterpret_cast<T*>(L.p + N);
+    o += (sizeof(Pose<T>) + sizeof(T)) * (size_t)N;
     o = align_up(o, 16);
     L.v = reinterpret_cast<T*>(base + o);
     o += sizeof(T) * 2 * (size_t)N * NV;
@@ -536,8 +540,8 @@ struct Evaluator {
     __device__ inline void rel(uint32_t e, int& i, int& j, T& dx, T& dy, bool& self) const {
         i = e & 0xfff;
         j = (e >> 12) & 0xfff;
-        dx = L.x[j] - L.x[i];
-        dy = L.y[j] - L.y[i];
+        dx = L.p[j].x - L.p[i].x;
+        dy = L.p[j].y - L.p[i].y;
         self = i == j;
         if (CRYSTAL) {
             const int nc = 2 * npd + 1;
@@ -600,13 +604,14 @@ struct Evaluator {
         float xi, yi, ti, ci, si, xj, yj, tj;
     };
     __device__ inline void lut_load(int i, int j, LutPose& o) const {
-        o.xi = L.x[i];
-        o.yi = L.y[i];
+        const Pose<T> a = L.p[i];
+        o.xi = a.x;
+        o.yi = a.y;
+        o.ci = a.c;
+        o.si = a.s;
         o.ti = L.t[i];
-        o.ci = L.c[i];
-        o.si = L.s[i];
-        o.xj = L.x[j];
-        o.yj = L.y[j];
+        o.xj = L.p[j].x;
+        o.yj = L.p[j].y;
         o.tj = L.t[j];
     }
     __device__ inline void lut_begin(int i, int j, const LutPose& o, LutPair& q) const {
@@ -669,7 +674,7 @@ struct Evaluator {
     // Piece-pair queue entry: i | j << 12 | pi << 24 | pj << 28.
     __device__ inline void clip_entry(uint32_t e) {
         const int i = e & 0xfff, j = (e >> 12) & 0xfff, pi = (e >> 24) & 0xf, pj = e >> 28;
-        piece_pair(i, j, pi, pj, L.x[j] - L.x[i], L.y[j] - L.y[i], false);
+        piece_pair(i, j, pi, pj, L.p[j].x - L.p[i].x, L.p[j].y - L.p[i].y, false);
     }
 
     // True when piece pi of tree i and piece pj of tree j (shifted by dx, dy) are separated
@@ -709,8 +714,8 @@ struct Evaluator {
                 const int pi = (int)(it >> 16);
                 const uint32_t e = tq[it & 0xffff];
                 const int i = e & 0xfff, j = (e >> 12) & 0xfff;
-                const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-                const T ci = L.c[i], si = L.s[i], cj = L.c[j], sj = L.s[j];
+                const T dx = L.p[j].x - L.p[i].x, dy = L.p[j].y - L.p[i].y;
+                const T ci = L.p[i].c, si = L.p[i].s, cj = L.p[j].c, sj = L.p[j].s;
                 const T cxa = ci * G.ccx[pi] - si * G.ccy[pi];
                 const T cya = si * G.ccx[pi] + ci * G.ccy[pi];
                 const T ra = G.cr[pi];
@@ -764,8 +769,8 @@ struct Evaluator {
                 const int pi = u - q * P;
                 const uint32_t e = tq[q];
                 const int i = e & 0xfff, j = (e >> 12) & 0xfff;
-                const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
-                const T ci = L.c[i], si = L.s[i];
+                const T dx = L.p[j].x - L.p[i].x, dy = L.p[j].y - L.p[i].y;
+                const T ci = L.p[i].c, si = L.p[i].s;
                 const T ox = ci * G.ccx[pi] - si * G.ccy[pi] - dx;
                 const T oy = si * G.ccx[pi] + ci * G.ccy[pi] - dy;
                 // tree j's bounding radius with a margin far above the rounding of the
@@ -873,7 +878,7 @@ struct Evaluator {
                 for (int u = 0; u < U; ++u) {
                     const int k = base + u * BLOCK + lane;
                     const int i = e[u] & 0xfff, j = (e[u] >> 12) & 0xfff;
-                    const T dx = L.x[j] - L.x[i], dy = L.y[j] - L.y[i];
+                    const T dx = L.p[j].x - L.p[i].x, dy = L.p[j].y - L.p[i].y;
                     pred[u] = k < nl && dx * dx + dy * dy <= G.cutoff_sq;
                 }
 #pragma unroll
@@ -896,8 +901,8 @@ struct Evaluator {
                 if (alive) {
                     ci = L.cell[i];
                     target = ci;
-                    xi = L.x[i];
-                    yi = L.y[i];
+                    xi = L.p[i].x;
+                    yi = L.p[i].y;
                     j = L.head[cell_bucket(ci, L.H)];
                 }
                 while (true) {
@@ -915,7 +920,7 @@ struct Evaluator {
                             const int jj = j;
                             j = L.next[jj];
                             if (L.cell[jj] == target && (s > 0 || jj > i)) {
-                                const T dx = L.x[jj] - xi, dy = L.y[jj] - yi;
+                                const T dx = L.p[jj].x - xi, dy = L.p[jj].y - yi;
                                 const T d2 = dx * dx + dy * dy;
                                 pred = d2 <= G.cutoff_sq;
                                 pl = d2 <= cut_list;
@@ -983,10 +988,10 @@ struct Evaluator {
                 T dx, dy;
                 bool self;
                 rel(L.tp[q], i, j, dx, dy, self);
-                const T cxa = L.c[i] * G.ccx[pi] - L.s[i] * G.ccy[pi];
-                const T cya = L.s[i] * G.ccx[pi] + L.c[i] * G.ccy[pi];
-                const T cxb = L.c[j] * G.ccx[pj] - L.s[j] * G.ccy[pj] + dx;
-                const T cyb = L.s[j] * G.ccx[pj] + L.c[j] * G.ccy[pj] + dy;
+                const T cxa = L.p[i].c * G.ccx[pi] - L.p[i].s * G.ccy[pi];
+                const T cya = L.p[i].s * G.ccx[pi] + L.p[i].c * G.ccy[pi];
+                const T cxb = L.p[j].c * G.ccx[pj] - L.p[j].s * G.ccy[pj] + dx;
+                const T cyb = L.p[j].s * G.ccx[pj] + L.p[j].c * G.ccy[pj] + dy;
                 const T rr = G.cr[pi] + G.cr[pj];
                 const T ddx = cxb - cxa, ddy = cyb - cya;
                 pred = ddx * ddx + ddy * ddy <= rr * rr;
@@ -1076,7 +1081,7 @@ struct Evaluator {
     }
 };
 
-// Stage poses (already in L.x/y/t) into sincos, rotated piece vertices (VERTS) and the
+// Stage poses (centres in L.p, angles in L.t) into sincos, rotated piece vertices (VERTS) and the
 // cell grid (GRID); clear the accumulators.  Ends with a barrier.
 template <typename T, int BLOCK, bool VERTS = true, bool GRID = true>
 __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
@@ -1084,8 +1089,8 @@ __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         T sn, cs;
         sincos_t(L.t[i], &sn, &cs);
-        L.c[i] = cs;
-        L.s[i] = sn;
+        L.p[i].c = cs;
+        L.p[i].s = sn;
     }
     for (int q = threadIdx.x; q < 3 * N + 1; q += BLOCK) L.g[q] = Acc<T>::zero();
     if (GRID)
@@ -1097,14 +1102,14 @@ __device__ inline void stage_instance(const GeomT<T>& G, EvalLds<T>& L) {
             const int i = q / NV;
             const int r = q - i * NV;
             const T lx = G.ux[r], ly = G.uy[r];
-            L.v[2 * q] = L.c[i] * lx - L.s[i] * ly;
-            L.v[2 * q + 1] = L.s[i] * lx + L.c[i] * ly;
+            L.v[2 * q] = L.p[i].c * lx - L.p[i].s * ly;
+            L.v[2 * q + 1] = L.p[i].s * lx + L.p[i].c * ly;
         }
     }
     if (GRID) {
         const T inv = grid_inv_side(G);
         for (int i = threadIdx.x; i < N; i += BLOCK) {
-            const uint32_t key = cell_key(L.x[i] * inv, L.y[i] * inv);
+            const uint32_t key = cell_key(L.p[i].x * inv, L.p[i].y * inv);
             L.cell[i] = key;
             L.next[i] = atomicExch(&L.head[cell_bucket(key, L.H)], i);
         }

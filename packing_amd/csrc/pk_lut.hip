@@ -125,8 +125,8 @@ __global__ __launch_bounds__(BLOCK) void k_lut_overlap(const float* __restrict__
     __syncthreads();
     EvalLds<float> L = carve_eval_lds<float>(smem, N, 0, false, BLOCK / WAVE);
     for (int i = threadIdx.x; i < N; i += BLOCK) {
-        L.x[i] = xyt[3 * (off + i) + 0];
-        L.y[i] = xyt[3 * (off + i) + 1];
+        L.p[i].x = xyt[3 * (off + i) + 0];
+        L.p[i].y = xyt[3 * (off + i) + 1];
         L.t[i] = xyt[3 * (off + i) + 2];
     }
     __syncthreads();

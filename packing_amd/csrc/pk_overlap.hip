@@ -47,8 +47,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((sizeof(T
     EvalLds<T> L = carve_eval_lds<T>(smem, N, G.NU, CRYSTAL, BLOCK / WAVE);
     L.P = G.P;
     for (int i = threadIdx.x; i < N; i += BLOCK) {
-        L.x[i] = xyt[3 * (off + i) + 0];
-        L.y[i] = xyt[3 * (off + i) + 1];
+        L.p[i].x = xyt[3 * (off + i) + 0];
+        L.p[i].y = xyt[3 * (off + i) + 1];
         L.t[i] = xyt[3 * (off + i) + 2];
     }
     __syncthreads();

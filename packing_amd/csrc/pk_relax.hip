@@ -81,13 +81,10 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
     const bool grid = list_mode != 2;
     for (int i = threadIdx.x; i < N; i += BLOCK) {
         const float ti = x[3 * i + 2];
-        L.x[i] = x[3 * i];
-        L.y[i] = x[3 * i + 1];
         L.t[i] = ti;
         float sn, cs;
         sincos_t(ti, &sn, &cs);
-        L.c[i] = cs;
-        L.s[i] = sn;
+        L.p[i] = Pose<float>{x[3 * i], x[3 * i + 1], cs, sn};
     }
     for (int q = threadIdx.x; q < 3 * N + 1; q += BLOCK) L.g[q] = 0ull;
     if (grid)
@@ -101,14 +98,14 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
             const int i = q / NV;
             const int r = q - i * NV;
             const float lx = G.ux[r], ly = G.uy[r];
-            L.v[2 * q] = L.c[i] * lx - L.s[i] * ly;
-            L.v[2 * q + 1] = L.s[i] * lx + L.c[i] * ly;
+            L.v[2 * q] = L.p[i].c * lx - L.p[i].s * ly;
+            L.v[2 * q + 1] = L.p[i].s * lx + L.p[i].c * ly;
         }
     }
     if (grid) {
         const float inv = grid_inv_side(G);
         for (int i = threadIdx.x; i < N; i += BLOCK) {
-            const uint32_t key = cell_key(L.x[i] * inv, L.y[i] * inv);
+            const uint32_t key = cell_key(L.p[i].x * inv, L.p[i].y * inv);
             L.cell[i] = key;
             L.next[i] = atomicExch(&L.head[cell_bucket(key, L.H)], i);
         }
@@ -144,7 +141,7 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
         int bn = 0;
         auto finish = [&](int i) {
             float gb[3], gh[3];
-            const float cb = boundary_tree<float, false>(G, L.x[i], L.y[i], L.c[i], L.s[i], half, hx, hy, gb, gh);
+            const float cb = boundary_tree<float, false>(G, L.p[i].x, L.p[i].y, L.p[i].c, L.p[i].s, half, hx, hy, gb, gh);
             lb[0] += Acc<float>::wide(cb);
             lb[1] += Acc<float>::wide(gh[0]);
             lb[2] += Acc<float>::wide(gh[1]);
@@ -157,7 +154,7 @@ __device__ __forceinline__ double relax_eval(const GeomT<float>& G, EvalLds<floa
             const int i = base + lane;
             bool near = false;
             if (i < N) {
-                near = !boundary_inside(G, L.x[i], L.y[i], half, hx, hy);
+                near = !boundary_inside(G, L.p[i].x, L.p[i].y, half, hx, hy);
                 if (HW && !near) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k)
