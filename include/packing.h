@@ -202,6 +202,13 @@ int pk_relax_step(pk_geometry_t g, const pk_relax_params* p, void* stream);
 int pk_relax_lds_bytes(pk_geometry_t g, int mode, int n_tree_max, int history, int hist_lds,
                        int block);
 
+/* Workgroups (instances) of pk_relax_step's kernel for these arguments that one CU holds at
+ * once (registers, LDS and waves), from the kernel and LDS size a launch would use; no
+ * launch.  No counterpart in the reference: the tile tuner records it beside each tile's
+ * measured times (how much of the GPU a bucket leaves to the others). */
+int pk_relax_occupancy(pk_geometry_t g, int mode, int n_tree_max, int history, int hist_lds,
+                       int block, int* blocks_per_cu);
+
 /*
  * Allocation-free relaxation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops,

@@ -47,6 +47,7 @@ _SIGS = {
     "pk_relax_run_noalloc": (i32, [vp, vp, i32, vp]),
     "pk_relax_slot_bytes": (i32, []),
     "pk_relax_lds_bytes": (i32, [vp, i32, i32, i32, i32, i32]),
+    "pk_relax_occupancy": (i32, [vp, i32, i32, i32, i32, i32, C.POINTER(i32)]),
     "pk_feasible": (i32, [vp, i32, vp, vp, i32, i32, vp, vp]),
     "pk_lbfgs_sys_bytes": (i32, []),
     "pk_lbfgs_begin": (i32, [vp, vp]),

@@ -6,11 +6,18 @@ and 5-9), or 64 / 128 / 256 threads per instance with the vectors in LDS.  Every
 or fixed-point), so the choice only affects speed.
 
 ``tune_batch`` times every candidate on each size bucket of a RelaxBatch (the bucket's
-own instances, from the batch's initial state, HIP events on the launch stream) and
-records the fastest in a table keyed by (mode, size class, pieces, history); the table
-is saved as ``tiles.json`` next to this file and read by ``choose_block``.  A size
-class is the register row count R = ceil((3N + 3) / 64) of the one-wave kernel up to
-its limit, then powers of two.  Without an entry the library's own rule applies (block
+own instances, from the batch's initial state, HIP events on the launch stream), then,
+since the buckets of a batch run concurrently and share the GPU, picks each size class's
+tile by the time of the batch's whole relaxation (all buckets at once, one class at a
+time with the others at their current choice): a tile that holds an instance in half the registers at
+a slightly longer launch of its own bucket leaves room for the other buckets (on the
+4096-instance table workload the one-wave kernel for the N <= 103 bucket, 0.98 ms per
+launch against 0.95 with two waves, took the batch from 12.4 to 11.4 ms).  The winners
+go into a table keyed by (mode, size class, pieces, history), saved as ``tiles.json``
+next to this file and read by ``choose_block``; each entry also records the bucket's own
+times and the workgroups per CU of each tile (``pk_relax_occupancy``).  A size class is
+the register row count R = ceil((3N + 3) / 64) of the one-wave kernel up to its limit,
+then powers of two.  Without an entry the library's own rule applies (block
 0: the one-wave kernel where it fits, else 256 threads).
 """
 from __future__ import annotations
@@ -61,6 +68,20 @@ def choose_block(mode: str, n_max: int, n_pieces: int, history: int = 8) -> int:
     return int(v) if v in BLOCKS else 0
 
 
+def occupancy(batch, n_max: int, blk: int) -> int:
+    """Instances of this bucket's kernel for tile ``blk`` that one CU holds at once."""
+    import ctypes as C
+    import torch
+    from . import _native as N
+    from .relax import MODES
+    o = batch.opts
+    nb = C.c_int(0)
+    with torch.cuda.device(batch.device):
+        N.check(N.lib().pk_relax_occupancy(batch._dg.handle, MODES[o.mode], int(n_max), int(o.history),
+                                           int(o.hist_lds), int(blk), C.byref(nb)), "pk_relax_occupancy")
+    return max(1, nb.value)
+
+
 def candidates(n_max: int, history: int):
     for blk in BLOCKS:
         if blk in (1, 2) and (n_max > WAVE_MAX_TREES or history > WAVE_MAX_HISTORY):
@@ -69,11 +90,27 @@ def candidates(n_max: int, history: int):
 
 
 def tune_batch(batch, iters: int = 20, reps: int = 2, save: bool = False, stream=None, path=None) -> list:
-    """Time every candidate tile on each size bucket of ``batch`` (a RelaxBatch): ``iters``
-    iterations from the batch's initial state, best of ``reps``.  Records the winners in
-    the table (and with ``save`` writes the table to ``path``, default tiles.json next to
-    this file); leaves the batch reset.  Returns one {key, n_max, instances, ms: {block:
-    ms}, block} per bucket."""
+    """Time every candidate tile on each size bucket of ``batch`` (a RelaxBatch, opts.block
+    0): ``iters`` iterations from the batch's initial state, best of ``reps``, for the
+    bucket alone, then (several buckets) the whole relaxation of the batch (opts.max_iter
+    iterations in launches of ``iters``) with each size class's candidates in turn.  Records the winners in the table (and with ``save`` writes the
+    table to ``path``, default tiles.json next to this file); leaves the batch reset.
+    Returns one {key, n_max, instances, ms: {block: ms}, batch_ms: {block: ms},
+    blocks_per_cu: {block: n}, block} per bucket."""
+
+    def timed(launch):
+        best = None
+        for _ in range(reps + 1):       # the first launch loads the code object
+            batch.reset()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            launch()
+            e1.record(st)
+            st.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        return round(best, 4)
+
     import torch
     st = stream if stream is not None else torch.cuda.Stream(device=batch.device)
     o = batch.opts
@@ -81,24 +118,36 @@ def tune_batch(batch, iters: int = 20, reps: int = 2, save: bool = False, stream
     out = []
     with torch.cuda.stream(st):
         for k, (_, cnt, nmax) in enumerate(batch.buckets()):
-            res = {}
+            res, occ = {}, {}
             for blk in candidates(nmax, o.history):
-                best = None
-                for _ in range(reps + 1):       # the first launch loads the code object
-                    batch.reset()
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record(st)
-                    batch.launch_bucket(k, iters, stream=st, block=blk)
-                    e1.record(st)
-                    st.synchronize()
-                    ms = e0.elapsed_time(e1)
-                    best = ms if best is None else min(best, ms)
-                res[blk] = round(best, 4)
+                occ[blk] = occupancy(batch, nmax, blk)
+                res[blk] = timed(lambda: batch.launch_bucket(k, iters, stream=st, block=blk))
             win = min(res, key=res.get)
             kk = key(o.mode, nmax, n_pieces, o.history)
-            table()[kk] = {"block": win, "ms": {str(b): v for b, v in res.items()}, "iters": iters,
+            table()[kk] = {"block": win, "ms": {str(b): v for b, v in res.items()},
+                           "blocks_per_cu": {str(b): v for b, v in occ.items()}, "iters": iters,
                            "instances": int(cnt)}
-            out.append({"key": kk, "n_max": int(nmax), "instances": int(cnt), "ms": res, "block": win})
+            out.append({"key": kk, "n_max": int(nmax), "instances": int(cnt), "ms": res, "blocks_per_cu": occ,
+                        "block": win})
+        if len(out) > 1:
+            chunks = -(-o.max_iter // iters)    # the whole relaxation, as the batch runs it
+            done = set()
+            for r in out:
+                kk = r["key"]
+                if kk in done:
+                    continue
+                done.add(kk)
+                bres = {}
+                for blk in r["ms"]:
+                    table()[kk]["block"] = blk
+                    bres[blk] = timed(lambda: batch.launch(iters, stream=st, chunks=chunks))
+                win = min(bres, key=bres.get)
+                table()[kk]["block"] = win
+                table()[kk]["batch_ms"] = {str(b): v for b, v in bres.items()}
+                for q in out:
+                    if q["key"] == kk:
+                        q["batch_ms"] = bres
+                        q["block"] = win
     batch.reset()
     torch.cuda.synchronize()
     if save:

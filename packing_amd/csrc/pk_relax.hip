@@ -1307,11 +1307,19 @@ size_t relax_w_lds_bytes(int mode, int n_max, int NV, int m, int R, int nwv) {
     return align_up(b, 16);
 }
 
+// Occupancy query (pk_relax_occupancy): while set, the launch helpers below store the
+// resident workgroups per CU of the kernel and LDS size they would launch, and launch nothing.
+static thread_local int* t_occupancy = nullptr;
+
 template <int MODE, int BLOCK, bool HL>
 static int launch_relax(const Geometry* g, const RelaxArgs& A, size_t lds, hipStream_t st) {
     const void* kern = A.p.history <= 8 ? reinterpret_cast<const void*>(k_relax<MODE, BLOCK, HL, 8>)
                                         : reinterpret_cast<const void*>(k_relax<MODE, BLOCK, HL, 16>);
     PK_CHECK_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (t_occupancy) {
+        PK_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(t_occupancy, kern, BLOCK, lds));
+        return 0;
+    }
     if (A.p.history <= 8)
         hipLaunchKernelGGL((k_relax<MODE, BLOCK, HL, 8>), dim3(A.p.n_launch), dim3(BLOCK), lds, st, A);
     else
@@ -1365,6 +1373,11 @@ static int launch_relax_w_rk(const Geometry* g, const RelaxArgs& A, hipStream_t 
     auto kern = k_relax_w<MODE, R, KH, NWV>;
     PK_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (t_occupancy) {
+        PK_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(t_occupancy, reinterpret_cast<const void*>(kern),
+                                                                  WAVE * NWV, lds));
+        return 0;
+    }
     hipLaunchKernelGGL(kern, dim3(A.p.n_launch), dim3(WAVE * NWV), lds, st, A);
     PK_CHECK_HIP(hipGetLastError());
     return 0;
@@ -1485,6 +1498,30 @@ using namespace pk;
 extern "C" {
 
 int pk_relax_slot_bytes(void) { return (int)sizeof(LbSlot); }
+
+int pk_relax_occupancy(pk_geometry_t h, int mode, int n_tree_max, int history, int hist_lds, int block,
+                       int* blocks_per_cu) {
+    const Geometry* g = reinterpret_cast<const Geometry*>(h);
+    PK_REQUIRE(g != nullptr && blocks_per_cu != nullptr, "pk_relax_occupancy: NULL argument");
+    PK_REQUIRE(mode == PK_MODE_AREA || mode == PK_MODE_SEP || mode == PK_MODE_LUT, "pk_relax_occupancy: bad mode");
+    PK_REQUIRE(history >= 1 && history <= PK_MAX_HISTORY, "pk_relax_occupancy: 1 <= history <= 16");
+    PK_REQUIRE(n_tree_max >= 1 && n_tree_max <= 4096, "pk_relax_occupancy: bad n_tree_max");
+    PK_REQUIRE(hist_lds >= -1 && hist_lds <= 1, "pk_relax_occupancy: hist_lds must be -1, 0 or 1");
+    RelaxArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.p.mode = mode;
+    A.p.n_tree_max = n_tree_max;
+    A.p.history = history;
+    A.p.hist_lds = hist_lds;
+    A.geom = g->df;
+    int nb = 0;
+    t_occupancy = &nb;
+    const int rc = relax_dispatch(g, A, block, nullptr);
+    t_occupancy = nullptr;
+    if (rc != PK_OK) return rc;
+    *blocks_per_cu = nb;
+    return PK_OK;
+}
 
 int pk_relax_lds_bytes(pk_geometry_t h, int mode, int n_tree_max, int history, int hist_lds, int block) {
     const Geometry* g = reinterpret_cast<const Geometry*>(h);

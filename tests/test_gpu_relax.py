@@ -373,3 +373,21 @@ def test_stop_rules_on_own_loss_sequence(oracle_mod, mode):
             else:
                 assert first is None or first > k_stop, (i, first, k_stop, reason)
     assert fired[16] >= 1 and fired[32] >= 1, fired
+
+
+def test_occupancy_query():
+    """pk_relax_occupancy reports the workgroups per CU of the kernel a launch would use: a
+    one-wave instance holds half the registers of a two-wave one (at most 5 rows per wave),
+    so a CU holds at least as many; unknown tiles are refused like in a launch."""
+    from packing_amd import autotune
+    geom = GEOMS["random"]()
+    xs, hs = ragged(3, [100])
+    b = batch(geom, xs, hs, mode="area", max_iter=10, history=8)
+    n1 = autotune.occupancy(b, 100, 1)
+    n2 = autotune.occupancy(b, 100, 2)
+    n256 = autotune.occupancy(b, 100, 256)
+    assert n1 >= 1 and n2 >= 1 and n256 >= 1
+    b = batch(geom, xs, hs, mode="sep", max_iter=10, history=8)
+    assert autotune.occupancy(b, 100, 1) >= autotune.occupancy(b, 100, 2) >= 1
+    with pytest.raises(ValueError):
+        autotune.occupancy(b, 100, 96)
