@@ -89,11 +89,13 @@ def candidates(n_max: int, history: int):
         yield blk
 
 
-def tune_batch(batch, iters: int = 20, reps: int = 2, save: bool = False, stream=None, path=None) -> list:
+def tune_batch(batch, iters: int = 20, reps: int = 2, save: bool = False, stream=None, path=None,
+               passes: int = 2) -> list:
     """Time every candidate tile on each size bucket of ``batch`` (a RelaxBatch, opts.block
     0): ``iters`` iterations from the batch's initial state, best of ``reps``, for the
     bucket alone, then (several buckets) the whole relaxation of the batch (opts.max_iter
-    iterations in launches of ``iters``) with each size class's candidates in turn.  Records the winners in the table (and with ``save`` writes the
+    iterations in launches of ``iters``) with each size class's candidates in turn
+    (``passes`` rounds over the classes).  Records the winners in the table (and with ``save`` writes the
     table to ``path``, default tiles.json next to this file); leaves the batch reset.
     Returns one {key, n_max, instances, ms: {block: ms}, batch_ms: {block: ms},
     blocks_per_cu: {block: n}, block} per bucket."""
@@ -129,8 +131,8 @@ def tune_batch(batch, iters: int = 20, reps: int = 2, save: bool = False, stream
                            "instances": int(cnt)}
             out.append({"key": kk, "n_max": int(nmax), "instances": int(cnt), "ms": res, "blocks_per_cu": occ,
                         "block": win})
-        if len(out) > 1:
-            chunks = -(-o.max_iter // iters)    # the whole relaxation, as the batch runs it
+        chunks = -(-o.max_iter // iters)        # the whole relaxation, as the batch runs it
+        for _ in range(passes if len(out) > 1 else 0):
             done = set()
             for r in out:
                 kk = r["key"]
